@@ -1,0 +1,290 @@
+// dense.hip -- the dense FP64 steps of the SCF iteration (scf/src/lib.rs:139-223).
+//
+// GEMM on the FP64 matrix cores: v_mfma_f64_16x16x4f64 through
+// __builtin_amdgcn_mfma_f64_16x16x4f64. A wavefront owns a 16x16 tile of C; per k-step of
+// 4 every lane feeds one element of A (row lane%16, k lane/16) and one of B (k lane/16,
+// column lane%16) and holds 4 accumulators C[4*r + lane/16][lane%16], r = 0..3.
+// Used for P = 2 C_occ C_occ^T, X^T F X, X C', F P S and the Loewdin transform.
+// The remaining helpers are elementwise or single-workgroup deterministic reductions.
+#include "kernels.hpp"
+
+namespace fock {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool transB, int M, int N, int K,
+                                                            double alpha, const double *__restrict__ A, int lda,
+                                                            const double *__restrict__ B, int ldb, double beta,
+                                                            double *__restrict__ C, int ldc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile_m = blockIdx.y * 2 + (wave >> 1), tile_n = blockIdx.x * 2 + (wave & 1);
+    const int row0 = tile_m * 16, col0 = tile_n * 16;
+    if (row0 >= M || col0 >= N) return;  // wave-uniform
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int ar = row0 + li, bc = col0 + li;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int kk = k0 + lk;
+        double a = 0.0, b = 0.0;
+        if (ar < M && kk < K) a = transA ? A[(size_t)kk * lda + ar] : A[(size_t)ar * lda + kk];
+        if (bc < N && kk < K) b = transB ? B[(size_t)bc * ldb + kk] : B[(size_t)kk * ldb + bc];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = row0 + 4 * r + lk, j = col0 + li;
+        if (i < M && j < N) {
+            double v = alpha * acc[r];
+            if (beta != 0.0) v += beta * C[(size_t)i * ldc + j];
+            C[(size_t)i * ldc + j] = v;
+        }
+    }
+}
+
+hipError_t gemm_f64(bool transA, bool transB, int m, int n, int k, double alpha, const double *A, int lda,
+                    const double *B, int ldb, double beta, double *C, int ldc, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    dim3 grid((n + 31) / 32, (m + 31) / 32), block(256);
+    hipLaunchKernelGGL(gemm_f64_mfma_kernel, grid, block, 0, s, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta,
+                       C, ldc);
+    return hipGetLastError();
+}
+
+__global__ void symmetrize_add_kernel(int n, const double *__restrict__ x, double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    out[e] = x[e] + x[(int64_t)j * n + i];
+}
+
+hipError_t symmetrize_add(int n, const double *x, double *out, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(symmetrize_add_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, x, out);
+    return hipGetLastError();
+}
+
+// G = (Jb + Jb^T) - scaleG (Kb + Kb^T); F = H + G (H may be null: F not written)
+__global__ void fock_from_jk_kernel(int n, const double *__restrict__ H, const double *__restrict__ Jb,
+                                    const double *__restrict__ Kb, double *__restrict__ G, double *__restrict__ F,
+                                    double scaleG) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    const int64_t t = (int64_t)j * n + i;
+    const double g = (Jb[e] + Jb[t]) - scaleG * (Kb[e] + Kb[t]);
+    G[e] = g;
+    if (F) F[e] = H[e] + g;
+}
+
+hipError_t fock_from_jk(int n, const double *H, const double *Jb, const double *Kb, double *G, double *F,
+                        double scaleG, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(fock_from_jk_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, H, Jb, Kb, G, F,
+                       scaleG);
+    return hipGetLastError();
+}
+
+// max of non-negative doubles through their bit patterns (monotone for x >= 0; NaN wins)
+__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void antisym_maxabs_kernel(int n, const double *__restrict__ M, double *__restrict__ e,
+                                      double *__restrict__ maxabs) {
+    __shared__ double red[256];
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (idx < (int64_t)n * n) {
+        const int i = (int)(idx / n), j = (int)(idx % n);
+        const double d = M[idx] - M[(int64_t)j * n + i];
+        if (e) e[idx] = d;
+        v = fabs(d);
+        if (d != d) v = d;  // keep NaN
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const double o = red[threadIdx.x + off];
+            const double m = red[threadIdx.x];
+            red[threadIdx.x] = (o != o || o > m) ? o : m;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomic_max_nonneg(maxabs, red[0]);
+}
+
+hipError_t antisym_maxabs(int n, const double *M, double *e, double *maxabs, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(antisym_maxabs_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, M, e, maxabs);
+    return hipGetLastError();
+}
+
+// one workgroup per dot product: deterministic reduction order
+__global__ __launch_bounds__(256) void frob_dots_kernel(int64_t len, const double *__restrict__ a,
+                                                        const double *__restrict__ base, int64_t stride,
+                                                        double *__restrict__ dots) {
+    __shared__ double red[256];
+    const double *b = base + (int64_t)blockIdx.x * stride;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < len; i += blockDim.x) acc += a[i] * b[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dots[blockIdx.x] = red[0];
+}
+
+hipError_t frob_dots(int64_t len, const double *a, const double *base, int64_t stride, int count, double *dots,
+                     hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(frob_dots_kernel, dim3(count), dim3(256), 0, s, len, a, base, stride, dots);
+    return hipGetLastError();
+}
+
+__global__ void lincomb_kernel(int64_t len, const double *__restrict__ base, int64_t stride,
+                               const int *__restrict__ slots, const double *__restrict__ c, int count,
+                               double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= len) return;
+    double acc = c[0] * base[(int64_t)slots[0] * stride + x];
+    for (int i = 1; i < count; ++i) acc = acc + c[i] * base[(int64_t)slots[i] * stride + x];
+    out[x] = acc;
+}
+
+hipError_t lincomb(int64_t len, const double *base, int64_t stride, const int *slots, const double *c, int count,
+                   double *out, hipStream_t s) {
+    hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, base, stride, slots,
+                       c, count, out);
+    return hipGetLastError();
+}
+
+hipError_t density_from_coeffs(int n, int nocc, const double *C, double *P, hipStream_t s) {
+    // P = 2 C[:, :nocc] C[:, :nocc]^T (scf/src/lib.rs:214-216)
+    if (nocc <= 0) return fill_f64((int64_t)n * n, 0.0, P, s);
+    return gemm_f64(false, true, n, n, nocc, 2.0, C, n, C, n, 0.0, P, n, s);
+}
+
+__global__ __launch_bounds__(256) void energy_trace_kernel(int n, const double *__restrict__ P,
+                                                           const double *__restrict__ H,
+                                                           const double *__restrict__ G, double *__restrict__ out) {
+    __shared__ double red[256];
+    const int64_t len = (int64_t)n * n;
+    double acc = 0.0;
+    for (int64_t e = threadIdx.x; e < len; e += blockDim.x) acc += P[e] * (2.0 * H[e] + G[e]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = 0.5 * red[0];
+}
+
+hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(energy_trace_kernel, dim3(1), dim3(256), 0, s, n, P, H, G, out);
+    return hipGetLastError();
+}
+
+__global__ void sub_kernel(int64_t len, const double *__restrict__ a, const double *__restrict__ b,
+                           double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < len) out[x] = a[x] - b[x];
+}
+hipError_t sub_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sub_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, a, b, out);
+    return hipGetLastError();
+}
+
+__global__ void fill_kernel(int64_t len, double v, double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < len) out[x] = v;
+}
+hipError_t fill_f64(int64_t len, double v, double *out, hipStream_t s) {
+    if (len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, v, out);
+    return hipGetLastError();
+}
+
+__global__ void copy_kernel(int64_t len, const double *__restrict__ in, double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < len) out[x] = in[x];
+}
+hipError_t copy_f64(int64_t len, const double *in, double *out, hipStream_t s) {
+    if (len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, in, out);
+    return hipGetLastError();
+}
+
+// scf/src/lib.rs:151-153: 0.875 * S_ij * (H_ii + H_jj), diagonal included
+__global__ void gwh_kernel(int n, const double *__restrict__ S, const double *__restrict__ H,
+                           double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    out[e] = 0.875 * S[e] * (H[(int64_t)i * n + i] + H[(int64_t)j * n + j]);
+}
+hipError_t gwh_guess(int n, const double *S, const double *H, double *out, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(gwh_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, S, H, out);
+    return hipGetLastError();
+}
+
+// out[i][j] = U[i][j] / sqrt(Dm[j][j])  (scf/src/lib.rs:143-144: diag(1/sqrt((U^T S U)_jj)))
+__global__ void scale_columns_kernel(int n, const double *__restrict__ U, const double *__restrict__ Dm, int ldd,
+                                     double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int j = (int)(e % n);
+    out[e] = U[e] * (1.0 / sqrt(Dm[(int64_t)j * ldd + j]));
+}
+hipError_t scale_columns(int n, const double *U, const double *Dm, int ldd, double *out, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(scale_columns_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, U, Dm, ldd, out);
+    return hipGetLastError();
+}
+
+// out[sa][sb] = max |P_ij| over the block of shells sa, sb (density-weighted screening)
+__global__ void shell_block_maxabs_kernel(int nshell, const int *__restrict__ f0, const int *__restrict__ nf, int n,
+                                          const double *__restrict__ P, float *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nshell * nshell) return;
+    const int sa = (int)(e / nshell), sb = (int)(e % nshell);
+    double m = 0.0;
+    for (int i = 0; i < nf[sa]; ++i)
+        for (int j = 0; j < nf[sb]; ++j) m = fmax(m, fabs(P[(int64_t)(f0[sa] + i) * n + f0[sb] + j]));
+    out[e] = (float)m;
+}
+hipError_t shell_block_maxabs(int nshell, const int *shell_f0, const int *shell_nf, int n, const double *P,
+                              float *out, hipStream_t s) {
+    const int64_t len = (int64_t)nshell * nshell;
+    hipLaunchKernelGGL(shell_block_maxabs_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, nshell,
+                       shell_f0, shell_nf, n, P, out);
+    return hipGetLastError();
+}
+
+__global__ void transpose_kernel(int n, const double *__restrict__ in, double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    out[(int64_t)j * n + i] = in[e];
+}
+hipError_t transpose_f64(int n, const double *in, double *out, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, in, out);
+    return hipGetLastError();
+}
+
+__global__ void add_kernel(int64_t len, const double *__restrict__ a, const double *__restrict__ b,
+                           double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < len) out[x] = a[x] + b[x];
+}
+hipError_t add_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(add_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, a, b, out);
+    return hipGetLastError();
+}
+
+}  // namespace fock

@@ -1,0 +1,288 @@
+// eri_kernels.hip -- direct-SCF shell-quartet ERI kernels with fused J/K digestion.
+//
+// Replaces the dense electron tensor (basis/src/electron_tensor.rs:81-176) and the
+// per-iteration n^4 contraction (scf/src/lib.rs:167-185) by a direct build.
+//
+// Work decomposition: one wavefront (64 lanes) per task. A task is one bra shell pair
+// and up to 64 ket shell pairs of one quartet class, one ket pair per lane. Every lane
+// of a wavefront shares the bra pair, so the bra primitive data are wave-uniform loads
+// and the J_ab contribution is reduced across the wavefront before a single atomic.
+// Ket pairs of a task have the same primitive-pair count (the host buckets them), so
+// the primitive loops are uniform and no lane idles inside them.
+//
+// Digestion of a unique quartet (ab|cd) with value v and degeneracy weight
+// w = (a==b ? 1/2 : 1)(c==d ? 1/2 : 1)(ab==cd ? 1/2 : 1) into non-symmetric buffers:
+//   J[a][b] += 2w v P[c][d]   J[c][d] += 2w v P[a][b]
+//   K[a][c] += w v P[b][d]    K[a][d] += w v P[b][c]   K[b][c] += w v P[a][d]   K[b][d] += w v P[a][c]
+// and J = J + J^T, K = K + K^T afterwards, which sums every one of the 8 symmetric
+// images of each integral exactly once.
+#include "fock_device.hpp"
+#include "kernels.hpp"
+
+namespace fock {
+
+template <int LA, int LB, int LC, int LD, int POLICY, int MODE, bool REFSCREEN, typename Real>
+__global__ __launch_bounds__(256) void eri_task_kernel(JKArgs args) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB, NCD = NC * ND;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wave >= args.ntasks) return;  // wave-uniform
+    const Task task = args.tasks[wave];
+    const bool active = lane < task.ket_count;
+    const PairInfo bra = args.pairs[task.bra];
+    const int ket_idx = task.ket_begin + (active ? lane : 0);
+    const PairInfo ket = args.pairs[ket_idx];
+    const PrimPair<Real> *__restrict__ pp = reinterpret_cast<const PrimPair<Real> *>(args.prims);
+
+    Real I[NAB][NCD];
+#pragma unroll
+    for (int x = 0; x < NAB; ++x)
+#pragma unroll
+        for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
+
+    for (int i = 0; i < bra.K; ++i) {
+        const PrimPair<Real> b = pp[bra.pp_start + i];
+        for (int k = 0; k < ket.K; ++k) {
+            const PrimPair<Real> kq = pp[ket.pp_start + k];
+            accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I);
+        }
+    }
+
+    const int n = args.nbf;
+    const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
+    if constexpr (REFSCREEN) {
+        // the reference's basis-function pair screen: keep (xy|zw) iff (xy|xy)(zw|zw) > thr,
+        // thr = the largest double whose square root does not exceed 1e-6
+        // (electron_tensor.rs:134-155); the diagonal integrals (xy|xy) themselves are computed
+        // unconditionally in its first pass (:89-101) and are never screened
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const double dab = args.D[(size_t)(fa + a) * n + (fb + b)];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) {
+                        const double dcd = args.D[(size_t)(fc + c) * n + (fd + d)];
+                        const bool diag = (fa + a == fc + c && fb + b == fd + d) || (fa + a == fd + d && fb + b == fc + c);
+                        if (!diag && !(dab * dcd > args.dthr)) I[a * NB + b][c * ND + d] = Real(0);
+                    }
+            }
+    }
+    if (!active) {
+#pragma unroll
+        for (int x = 0; x < NAB; ++x)
+#pragma unroll
+            for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
+    }
+
+    if constexpr (MODE == MODE_DENSE) {
+        if (!active) return;
+        double *out = args.dense;
+        const size_t N = (size_t)n;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) {
+                        const size_t i = fa + a, j = fb + b, k = fc + c, l = fd + d;
+                        const double v = (double)I[a * NB + b][c * ND + d];
+                        out[((i * N + j) * N + k) * N + l] = v;
+                        out[((j * N + i) * N + k) * N + l] = v;
+                        out[((i * N + j) * N + l) * N + k] = v;
+                        out[((j * N + i) * N + l) * N + k] = v;
+                        out[((k * N + l) * N + i) * N + j] = v;
+                        out[((l * N + k) * N + i) * N + j] = v;
+                        out[((k * N + l) * N + j) * N + i] = v;
+                        out[((l * N + k) * N + j) * N + i] = v;
+                    }
+        return;
+    } else {
+        Real w = Real(1);
+        if (fa == fb) w *= Real(0.5);
+        if (fc == fd) w *= Real(0.5);
+        if (task.bra == ket_idx) w *= Real(0.5);
+        const double *__restrict__ P = args.P;
+        double *__restrict__ J = args.J;
+        double *__restrict__ K = args.K;
+        const size_t N = (size_t)n;
+
+        // J[a][b]: wave-uniform destination, reduce over lanes first
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                Real s = Real(0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fc + c) * N + (fd + d)];
+                double sd = wave_sum((double)(s * w));
+                if (lane == 0) unsafeAtomicAdd(&J[(fa + a) * N + (fb + b)], 2.0 * sd);
+            }
+        if (!active) return;
+        // J[c][d]
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                Real s = Real(0);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fb + b)];
+                unsafeAtomicAdd(&J[(fc + c) * N + (fd + d)], 2.0 * (double)(s * w));
+            }
+        // K[a][c] += sum_bd v P[b][d]
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                Real s = Real(0);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fd + d)];
+                unsafeAtomicAdd(&K[(fa + a) * N + (fc + c)], (double)(s * w));
+            }
+        // K[a][d] += sum_bc v P[b][c]
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                Real s = Real(0);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fc + c)];
+                unsafeAtomicAdd(&K[(fa + a) * N + (fd + d)], (double)(s * w));
+            }
+        // K[b][c] += sum_ad v P[a][d]
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                Real s = Real(0);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fd + d)];
+                unsafeAtomicAdd(&K[(fb + b) * N + (fc + c)], (double)(s * w));
+            }
+        // K[b][d] += sum_ac v P[a][c]
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                Real s = Real(0);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fc + c)];
+                unsafeAtomicAdd(&K[(fb + b) * N + (fd + d)], (double)(s * w));
+            }
+    }
+}
+
+// (ab|ab) for every shell pair of one class, one thread per pair: the diagonal the
+// reference computes first (basis/src/electron_tensor.rs:89-101) and from which both its
+// 1e-6 screen and the Schwarz bounds are taken. Writes D[x][y] = D[y][x] = (xy|xy).
+template <int LA, int LB, int POLICY>
+__global__ __launch_bounds__(256) void eri_diag_kernel(DiagArgs args) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= args.npairs) return;
+    const PairInfo pr = args.pairs[args.first + t];
+    const PrimPair<double> *__restrict__ pp = args.prims;
+    double I[NAB][NAB];
+#pragma unroll
+    for (int x = 0; x < NAB; ++x)
+#pragma unroll
+        for (int y = 0; y < NAB; ++y) I[x][y] = 0.0;
+    for (int i = 0; i < pr.K; ++i) {
+        const PrimPair<double> b = pp[pr.pp_start + i];
+        for (int k = 0; k < pr.K; ++k) {
+            const PrimPair<double> kq = pp[pr.pp_start + k];
+            accumulate_prim_quartet<LA, LB, LA, LB, POLICY, double>(b, kq, args.bt, I);
+        }
+    }
+    const size_t N = (size_t)args.nbf;
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const double v = I[a * NB + b][a * NB + b];
+            args.D[(pr.fa + a) * N + (pr.fb + b)] = v;
+            args.D[(pr.fb + b) * N + (pr.fa + a)] = v;
+        }
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <int LA, int LB, int LC, int LD, int POLICY, int MODE, bool REFSCREEN, typename Real>
+static hipError_t launch_one(const JKArgs &a, hipStream_t s) {
+    if (a.ntasks <= 0) return hipSuccess;
+    dim3 grid((a.ntasks + 3) / 4), block(256);
+    hipLaunchKernelGGL((eri_task_kernel<LA, LB, LC, LD, POLICY, MODE, REFSCREEN, Real>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD, int MODE>
+static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s) {
+    (void)cls;
+    if (fp32) {
+        if (policy != 1) return hipErrorInvalidValue;
+        if constexpr (MODE == MODE_JK) {
+            return refscreen ? launch_one<LA, LB, LC, LD, 1, MODE, true, float>(a, s)
+                             : launch_one<LA, LB, LC, LD, 1, MODE, false, float>(a, s);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    if (policy == 0)
+        return refscreen ? launch_one<LA, LB, LC, LD, 0, MODE, true, double>(a, s)
+                         : launch_one<LA, LB, LC, LD, 0, MODE, false, double>(a, s);
+    return refscreen ? launch_one<LA, LB, LC, LD, 1, MODE, true, double>(a, s)
+                     : launch_one<LA, LB, LC, LD, 1, MODE, false, double>(a, s);
+}
+
+template <int MODE>
+static hipError_t dispatch_class(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s) {
+    switch (cls) {
+        case 0: return dispatch_policy<0, 0, 0, 0, MODE>(cls, a, policy, refscreen, fp32, s);
+        case 1: return dispatch_policy<1, 0, 0, 0, MODE>(cls, a, policy, refscreen, fp32, s);
+        case 2: return dispatch_policy<1, 0, 1, 0, MODE>(cls, a, policy, refscreen, fp32, s);
+        case 3: return dispatch_policy<1, 1, 0, 0, MODE>(cls, a, policy, refscreen, fp32, s);
+        case 4: return dispatch_policy<1, 1, 1, 0, MODE>(cls, a, policy, refscreen, fp32, s);
+        case 5: return dispatch_policy<1, 1, 1, 1, MODE>(cls, a, policy, refscreen, fp32, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_eri_tasks(int cls, int mode, const JKArgs &a, int policy, bool refscreen, bool fp32,
+                            hipStream_t s) {
+    if (mode == MODE_DENSE) return dispatch_class<MODE_DENSE>(cls, a, policy, refscreen, fp32, s);
+    return dispatch_class<MODE_JK>(cls, a, policy, refscreen, fp32, s);
+}
+
+hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s) {
+    if (a.npairs <= 0) return hipSuccess;
+    dim3 grid((a.npairs + 255) / 256), block(256);
+    switch (pair_class * 2 + (policy ? 1 : 0)) {
+        case 0: hipLaunchKernelGGL((eri_diag_kernel<0, 0, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((eri_diag_kernel<0, 0, 1>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((eri_diag_kernel<1, 0, 0>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((eri_diag_kernel<1, 0, 1>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((eri_diag_kernel<1, 1, 0>), grid, block, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((eri_diag_kernel<1, 1, 1>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fock

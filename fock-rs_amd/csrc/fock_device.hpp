@@ -1,0 +1,422 @@
+// fock_device.hpp -- device building blocks shared by the integral kernels.
+//
+// McMurchie-Davidson integrals over Cartesian Gaussians in the reference's
+// conventions: a basis function is (A - r)^{ijk} exp(-a |r - A|^2)
+// (basis/src/contracted_gaussian.rs:320-331), Hermite coefficients are expanded
+// with dist = B - A (basis/src/utils.rs:33-50, called from
+// contracted_gaussian.rs:172-182) and the Hermite Coulomb integrals R_{tuv} are
+// evaluated at Q - P, ket centre minus bra centre (contracted_gaussian.rs:294).
+//
+// Policies (compile-time template parameters, one kernel body):
+//   BoysRef     the reference's table F_10(0.15k) with linear interpolation and
+//               downward recursion (boys/src/lib.rs:58-81)
+//   BoysTaylor  Taylor expansion about the nearest point of a 0.1-spaced grid of
+//               accurate F_m(T_i), downward recursion, asymptotic form for T >= 45
+//   RBug        the reference's x-branch R^n_{t,u,v} = X R^{n+1}_{t-1} + R^{n+1}_{t-2}
+//               without the (t-1) factor (basis/src/utils.rs:98-105)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+namespace fock {
+
+constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i in [Begin, End).
+template <int Begin, int End, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (Begin < End) {
+        f(std::integral_constant<int, Begin>{});
+        static_for<Begin + 1, End>(f);
+    }
+}
+
+// Cartesian exponent (axis 0 = x, 1 = y, 2 = z) of component c of a shell with angular
+// momentum L, in the reference's order: for x in 0..=L, y in 0..=L, z = L - x - y >= 0
+// (basis_set/src/lib.rs:87-99).
+constexpr int cart(int L, int c, int axis) {
+    int idx = 0;
+    for (int x = 0; x <= L; ++x)
+        for (int y = 0; y <= L; ++y) {
+            int z = L - x - y;
+            if (z < 0) continue;
+            if (idx == c) return axis == 0 ? x : (axis == 1 ? y : z);
+            ++idx;
+        }
+    return -1;
+}
+
+// ------------------------------------------------------------------ data layout
+
+// One primitive pair (a, b) of a shell pair, precomputed on the host.
+// Kc = c_a c_b exp(-mu X^2) exp(-mu Y^2) exp(-mu Z^2) / p carries the contraction
+// coefficients, the Gaussian product prefactor (the t=0 Hermite coefficient of the
+// reference is exp(-q dist^2) per axis, utils.rs:39-40) and 1/p of the ERI prefactor.
+// xa = A - P and xb = B - P are the reference's recursion factors -(q dist / a) and
+// +(q dist / b) (utils.rs:42-48).
+template <typename Real>
+struct alignas(sizeof(Real) * 4) PrimPair {
+    Real p, Px, Py, Pz;
+    Real Kc, xax, xay, xaz;
+    Real xbx, xby, xbz, inv2p;
+};
+
+// Shell pair: primitive-pair range, first basis functions of both shells.
+struct PairInfo {
+    int pp_start, K, fa, fb;
+};
+
+// One wavefront task: bra pair, range of ket pairs (one per lane).
+struct Task {
+    int bra, ket_begin, ket_count, flags;
+};
+
+// ------------------------------------------------------------------ Boys
+
+constexpr int BOYS_GRID_N = 451;        // T_i = 0.1 i, i = 0..450
+constexpr double BOYS_GRID_STEP = 0.1;
+constexpr double BOYS_TMAX = 45.0;      // asymptotic regime beyond (e^-T / F_n < 1e-14)
+constexpr int BOYS_ORDERS = 20;         // tabulated orders m = 0..19 per grid point
+constexpr int BOYS_TAYLOR64 = 8;        // terms (k = 0..7): |error| < 0.05^8/8! F_n ~ 1e-15 F_n
+constexpr int BOYS_TAYLOR32 = 5;        // terms (k = 0..4): < 0.05^5/5! ~ 3e-9 relative
+
+struct BoysTables {
+    const double *lut_ref;  // the reference's 200-entry F_10 table
+    const double *tab64;    // [BOYS_GRID_N][BOYS_ORDERS]
+    const float *tab32;     // [BOYS_GRID_N][BOYS_ORDERS]
+};
+
+__device__ __forceinline__ double rcp_exact(double x) { return 1.0 / x; }
+
+// (2n-1)!!, n >= 0 (boys/src/lib.rs:5-10 with the saturating 2n-1 of :67)
+__host__ __device__ constexpr double dfact_odd(int n) {
+    double p = 1.0;
+    for (int k = 2 * n - 1; k > 1; k -= 2) p *= k;
+    return p;
+}
+
+// powi with the square-and-multiply order of Rust's f64::powi
+__device__ __forceinline__ double powi_ref(double a, int b) {
+    const bool recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+// F_0..F_L at T with the reference's algorithm (boys/src/lib.rs:58-81). One downward pass
+// from order 10 yields every lower order with the same operations the reference performs in
+// its separate calls boys(t, n), n = 0..L.
+template <int L>
+__device__ __forceinline__ void boys_ref(double T, const double *__restrict__ lut, double F[L + 1]) {
+    static_assert(L <= 10, "the reference tabulates orders up to 10");
+    if (fabs(T) < 1e-16) {
+#pragma unroll
+        for (int n = 0; n <= L; ++n) F[n] = 1.0 / (double)(2 * n + 1);
+    } else if (fabs(T) >= 30.0) {
+#pragma unroll
+        for (int n = 0; n <= L; ++n)
+            F[n] = dfact_odd(n) * (1.0 / (double)(1 << (n + 1))) * sqrt(M_PI * powi_ref(T, -2 * n - 1));
+    } else {
+        double index_f = 200.0 * T / 30.0;
+        double lower = floor(index_f);
+        double upper = ceil(index_f);
+        double int_t = index_f - lower;
+        // `as usize` saturates (negative or NaN -> 0); upper is clamped to N-1 (lib.rs:76-78)
+        int lo = (int)fmin(fmax(lower, 0.0), 199.0);
+        int up = (int)fmin(fmax(upper, 0.0), 199.0);
+        double curr = int_t * lut[up] + (1.0 - int_t) * lut[lo];
+        double e = exp(-T);
+        double t2 = 2.0 * T;
+#pragma unroll
+        for (int k = 10; k > 0; --k) {
+            if (k <= L) F[k] = curr;
+            curr = (t2 * curr + e) * (1.0 / (double)(2 * k - 1));
+        }
+        F[0] = curr;
+    }
+}
+
+// Accurate F_0..F_L (FP64): Taylor expansion about the nearest grid point.
+// Low total order: every order from its own expansion; higher: the top order from the
+// expansion and the rest by downward recursion with exp(-T) = exp(-T_i) exp(-dT).
+template <int L>
+__device__ __forceinline__ void boys_taylor64(double T, const double *__restrict__ tab, double F[L + 1]) {
+    if (T < BOYS_TMAX) {
+        int i = (int)fmax(T * 10.0 + 0.5, 0.0);  // T >= 0 for every integral argument
+        double d = (double)i * BOYS_GRID_STEP - T;  // -dT
+        const double *row = tab + i * BOYS_ORDERS;
+        if constexpr (L <= 2) {
+#pragma unroll
+            for (int n = 0; n <= L; ++n) {
+                double acc = row[n + BOYS_TAYLOR64 - 1];
+#pragma unroll
+                for (int k = BOYS_TAYLOR64 - 1; k > 0; --k) acc = fma(acc, d * (1.0 / k), row[n + k - 1]);
+                F[n] = acc;
+            }
+        } else {
+            double acc = row[L + BOYS_TAYLOR64 - 1];
+#pragma unroll
+            for (int k = BOYS_TAYLOR64 - 1; k > 0; --k) acc = fma(acc, d * (1.0 / k), row[L + k - 1]);
+            F[L] = acc;
+            // exp(-T) = exp(-T_i) * exp(d); the grid value exp(-T_i) sits in the last column
+            double ed = 1.0 + d * (1.0 + d * (0.5 + d * (1.0 / 6 + d * (1.0 / 24 + d * (1.0 / 120 + d * (1.0 / 720 + d * (1.0 / 5040)))))));
+            double e = row[BOYS_ORDERS - 1] * ed;
+            double t2 = 2.0 * T;
+#pragma unroll
+            for (int n = L; n > 0; --n) F[n - 1] = fma(t2, F[n], e) * (1.0 / (2 * n - 1));
+        }
+    } else {
+        // upward recursion without the exp(-T) term: relative error below 1e-14 for the
+        // orders of s/p quartets (L <= 4) at T >= 45
+        static_assert(L <= 4, "extend the grid or keep exp(-T) for higher orders");
+        double r = rsqrt(T);
+        F[0] = 0.886226925452758 * r;  // sqrt(pi)/2 / sqrt(T)
+        double h = 0.5 * r * r;        // 1/(2T)
+#pragma unroll
+        for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((2 * n - 1) * h);
+    }
+}
+
+// Accurate F_0..F_L in FP32 (mixed precision); large T in FP64 for the asymptotic powers.
+template <int L>
+__device__ __forceinline__ void boys_taylor32(float T, const float *__restrict__ tab, float F[L + 1]) {
+    if (T < (float)BOYS_TMAX) {
+        int i = (int)fmaxf(T * 10.0f + 0.5f, 0.0f);
+        float d = (float)i * 0.1f - T;
+        const float *row = tab + i * BOYS_ORDERS;
+        if constexpr (L <= 2) {
+#pragma unroll
+            for (int n = 0; n <= L; ++n) {
+                float acc = row[n + BOYS_TAYLOR32 - 1];
+#pragma unroll
+                for (int k = BOYS_TAYLOR32 - 1; k > 0; --k) acc = fmaf(acc, d * (1.0f / k), row[n + k - 1]);
+                F[n] = acc;
+            }
+        } else {
+            float acc = row[L + BOYS_TAYLOR32 - 1];
+#pragma unroll
+            for (int k = BOYS_TAYLOR32 - 1; k > 0; --k) acc = fmaf(acc, d * (1.0f / k), row[L + k - 1]);
+            F[L] = acc;
+            float ed = 1.0f + d * (1.0f + d * (0.5f + d * (1.0f / 6 + d * (1.0f / 24))));
+            float e = row[BOYS_ORDERS - 1] * ed;
+            float t2 = 2.0f * T;
+#pragma unroll
+            for (int n = L; n > 0; --n) F[n - 1] = fmaf(t2, F[n], e) * (1.0f / (2 * n - 1));
+        }
+    } else {
+        float r = rsqrtf(T);
+        F[0] = 0.886226925452758f * r;
+        float h = 0.5f * r * r;
+#pragma unroll
+        for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((2 * n - 1) * h);
+    }
+}
+
+// ------------------------------------------------------------------ Hermite
+
+// 1-D Hermite coefficients E[i][j][t] (i <= LA, j <= LB) of one primitive pair along one
+// axis, without the exp factor (carried in Kc). Recursion of basis/src/utils.rs:33-50:
+// j > 0 lowers j with factor xb, j == 0 lowers i with factor xa.
+template <int LA, int LB, typename Real>
+__device__ __forceinline__ void hermite_1d(Real xa, Real xb, Real inv2p, Real E[LA + 1][LB + 1][LA + LB + 1]) {
+#pragma unroll
+    for (int i = 0; i <= LA; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB; ++j)
+#pragma unroll
+            for (int t = 0; t <= LA + LB; ++t) E[i][j][t] = Real(0);
+    E[0][0][0] = Real(1);
+#pragma unroll
+    for (int i = 0; i <= LA; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB; ++j) {
+            if (i == 0 && j == 0) continue;
+#pragma unroll
+            for (int t = 0; t <= i + j; ++t) {
+                Real v;
+                if (j > 0) {
+                    v = xb * E[i][j - 1][t];
+                    if (t + 1 <= i + j - 1) v += Real(t + 1) * E[i][j - 1][t + 1];
+                    if (t > 0) v += inv2p * E[i][j - 1][t - 1];
+                } else {
+                    v = xa * E[i - 1][j][t];
+                    if (t + 1 <= i + j - 1) v += Real(t + 1) * E[i - 1][j][t + 1];
+                    if (t > 0) v += inv2p * E[i - 1][j][t - 1];
+                }
+                E[i][j][t] = v;
+            }
+        }
+}
+
+// Hermite Coulomb integrals R^{(0)}_{tuv}, t+u+v <= L, from Fs[n] = (-2 alpha)^n F_n(T),
+// with the reference's branch order (basis/src/utils.rs:82-105): t > 0 -> x branch,
+// else u > 0 -> y branch, else z branch. RBUG drops the (t-1) factor of the x branch.
+template <int L, bool RBUG, typename Real>
+__device__ __forceinline__ void hermite_coulomb(const Real Fs[L + 1], Real X, Real Y, Real Z,
+                                                Real R[L + 1][L + 1][L + 1]) {
+    R[0][0][0] = Fs[L];
+#pragma unroll
+    for (int n = L - 1; n >= 0; --n) {
+#pragma unroll
+        for (int tot = L - n; tot >= 1; --tot) {
+#pragma unroll
+            for (int t = 0; t <= tot; ++t)
+#pragma unroll
+                for (int u = 0; u <= tot - t; ++u) {
+                    const int v = tot - t - u;
+                    Real val;
+                    if (t > 0) {
+                        val = X * R[t - 1][u][v];
+                        if (t > 1) val += (RBUG ? Real(1) : Real(t - 1)) * R[t - 2][u][v];
+                    } else if (u > 0) {
+                        val = Y * R[t][u - 1][v];
+                        if (u > 1) val += Real(u - 1) * R[t][u - 2][v];
+                    } else {
+                        val = Z * R[t][u][v - 1];
+                        if (v > 1) val += Real(v - 1) * R[t][u][v - 2];
+                    }
+                    R[t][u][v] = val;
+                }
+        }
+        R[0][0][0] = Fs[n];
+    }
+}
+
+// ------------------------------------------------------------------ quartet body
+
+// Accumulate one primitive quartet into the contracted integrals I[ab][cd] (bra components
+// a-major, ket components c-major). POLICY: 0 = reference arithmetic (BoysRef + RBug),
+// 1 = accurate. The bra and ket primitive pairs are passed by value.
+template <int LA, int LB, int LC, int LD, int POLICY, typename Real>
+__device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &bra, const PrimPair<Real> &ket,
+                                                        const BoysTables &bt,
+                                                        Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)]) {
+    constexpr int L = LA + LB + LC + LD;
+    constexpr int LAB = LA + LB;
+    constexpr int LCD = LC + LD;
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    const Real p = bra.p, q = ket.p;
+    const Real pq = p + q;
+    Real rs, alpha;
+    if constexpr (sizeof(Real) == 8) {
+        rs = rsqrt(pq);
+    } else {
+        rs = rsqrtf(pq);
+    }
+    alpha = p * q * (rs * rs);
+    // comp_diff = Q - P (contracted_gaussian.rs:294)
+    const Real X = ket.Px - bra.Px, Y = ket.Py - bra.Py, Z = ket.Pz - bra.Pz;
+    const Real T = alpha * (X * X + Y * Y + Z * Z);
+    // 2 pi^{5/2} / (p q sqrt(p+q)) with 1/p, 1/q inside Kc
+    const Real pref = Real(34.98683665524972) * bra.Kc * ket.Kc * rs;
+    Real Fs[L + 1];
+    if constexpr (POLICY == 0) {
+        double Fd[L + 1];
+        boys_ref<L>((double)T, bt.lut_ref, Fd);
+#pragma unroll
+        for (int n = 0; n <= L; ++n) Fs[n] = (Real)Fd[n];
+    } else if constexpr (sizeof(Real) == 8) {
+        boys_taylor64<L>(T, bt.tab64, Fs);
+    } else {
+        boys_taylor32<L>(T, bt.tab32, Fs);
+    }
+    {
+        const Real m2a = Real(-2) * alpha;
+        Real f = m2a;
+#pragma unroll
+        for (int n = 1; n <= L; ++n) {
+            Fs[n] *= f;
+            f *= m2a;
+        }
+    }
+    Real R[L + 1][L + 1][L + 1];
+    hermite_coulomb<L, POLICY == 0, Real>(Fs, X, Y, Z, R);
+
+    Real Eab[3][LA + 1][LB + 1][LAB + 1];
+    Real Ecd[3][LC + 1][LD + 1][LCD + 1];
+    if constexpr (LAB > 0) {
+        hermite_1d<LA, LB, Real>(bra.xax, bra.xbx, bra.inv2p, Eab[0]);
+        hermite_1d<LA, LB, Real>(bra.xay, bra.xby, bra.inv2p, Eab[1]);
+        hermite_1d<LA, LB, Real>(bra.xaz, bra.xbz, bra.inv2p, Eab[2]);
+    } else {
+        Eab[0][0][0][0] = Eab[1][0][0][0] = Eab[2][0][0][0] = Real(1);
+    }
+    if constexpr (LCD > 0) {
+        hermite_1d<LC, LD, Real>(ket.xax, ket.xbx, ket.inv2p, Ecd[0]);
+        hermite_1d<LC, LD, Real>(ket.xay, ket.xby, ket.inv2p, Ecd[1]);
+        hermite_1d<LC, LD, Real>(ket.xaz, ket.xbz, ket.inv2p, Ecd[2]);
+    } else {
+        Ecd[0][0][0][0] = Ecd[1][0][0][0] = Ecd[2][0][0][0] = Real(1);
+    }
+
+    static_for<0, NC>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
+        constexpr int cx = cart(LC, c, 0), cy = cart(LC, c, 1), cz = cart(LC, c, 2);
+        static_for<0, ND>([&](auto d_) {
+            constexpr int d = decltype(d_)::value;
+            constexpr int dx = cart(LD, d, 0), dy = cart(LD, d, 1), dz = cart(LD, d, 2);
+            // ket contraction Xk[t][u][v] = sum (-1)^{tt+uu+vv} E^cd_{tt,uu,vv} R_{t+tt,u+uu,v+vv}
+            // over the bra Hermite simplex t+u+v <= LA+LB
+            Real Xk[LAB + 1][LAB + 1][LAB + 1];
+            static_for<0, LAB + 1>([&](auto t_) {
+                constexpr int t = decltype(t_)::value;
+                static_for<0, LAB + 1 - t>([&](auto u_) {
+                    constexpr int u = decltype(u_)::value;
+                    static_for<0, LAB + 1 - t - u>([&](auto v_) {
+                        constexpr int v = decltype(v_)::value;
+                        Real acc = Real(0);
+                        static_for<0, cx + dx + 1>([&](auto tt_) {
+                            constexpr int tt = decltype(tt_)::value;
+                            static_for<0, cy + dy + 1>([&](auto uu_) {
+                                constexpr int uu = decltype(uu_)::value;
+                                static_for<0, cz + dz + 1>([&](auto vv_) {
+                                    constexpr int vv = decltype(vv_)::value;
+                                    Real e = Ecd[0][cx][dx][tt] * Ecd[1][cy][dy][uu] * Ecd[2][cz][dz][vv];
+                                    if constexpr ((tt + uu + vv) & 1) acc -= e * R[t + tt][u + uu][v + vv];
+                                    else acc += e * R[t + tt][u + uu][v + vv];
+                                });
+                            });
+                        });
+                        Xk[t][u][v] = acc;
+                    });
+                });
+            });
+            static_for<0, NA>([&](auto a_) {
+                constexpr int a = decltype(a_)::value;
+                constexpr int ax = cart(LA, a, 0), ay = cart(LA, a, 1), az = cart(LA, a, 2);
+                static_for<0, NB>([&](auto b_) {
+                    constexpr int b = decltype(b_)::value;
+                    constexpr int bx = cart(LB, b, 0), by = cart(LB, b, 1), bz = cart(LB, b, 2);
+                    Real acc = Real(0);
+                    static_for<0, ax + bx + 1>([&](auto t_) {
+                        constexpr int t = decltype(t_)::value;
+                        static_for<0, ay + by + 1>([&](auto u_) {
+                            constexpr int u = decltype(u_)::value;
+                            static_for<0, az + bz + 1>([&](auto v_) {
+                                constexpr int v = decltype(v_)::value;
+                                acc += Eab[0][ax][bx][t] * Eab[1][ay][by][u] * Eab[2][az][bz][v] * Xk[t][u][v];
+                            });
+                        });
+                    });
+                    I[a * NB + b][c * ND + d] += pref * acc;
+                });
+            });
+        });
+    });
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace fock

@@ -1,0 +1,1111 @@
+// fock_host.cpp -- the C ABI (include/fock.h): context, basis preparation, direct J/K
+// builds and the SCF driver.
+//
+// Host responsibilities: group the reference's flat list of Cartesian functions into
+// shells, precompute primitive-pair data, compute the diagonal integrals (ab|ab) on the
+// GPU, build deterministic screened task lists, and run the SCF iteration of
+// scf/src/lib.rs:73-258 with every matrix resident in HBM. Only scalars (the DIIS
+// error, its dot products, the energy) cross to the host per iteration.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fock.h"
+#include "kernels.hpp"
+
+using namespace fock;
+
+namespace {
+
+const double REF_LUT_HOST[200] = {
+#include "ref_boys_lut.inc"
+};
+
+// f_prim of the canonical McMurchie-Davidson count per quartet class (see fock_jk_stats)
+constexpr double F_PRIM[NUM_CLASSES] = {38, 85, 266, 298, 977, 3833};
+constexpr int CLASS_NINT[NUM_CLASSES] = {1, 3, 9, 9, 27, 81};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t b) {
+        if (b <= bytes && p) return hipSuccess;
+        release();
+        if (b == 0) b = 8;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <typename T>
+    T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Shell {
+    int L, f0, nf, K, prim0;  // prim0: index of the first primitive of function f0
+    double A[3];
+};
+
+struct PairHost {
+    int sa, sb, cls, K, pp_start, fa, fb, creation;
+    double dmax;  // max over components of (ab|ab)
+};
+
+struct Prepared {
+    bool valid = false;
+    int n = 0, policy = 1, screen = 1, precision = 0;
+    double tau = 1e-12, dthr = 0.0;
+    std::vector<Shell> shells;
+    std::vector<PairHost> pairs;  // in task (sorted) order
+    DevBuf d_pairs, d_prims64, d_prims32, d_D, d_tasks;
+    int task_off[NUM_CLASSES + 1] = {0};
+    fock_jk_stats stats{};
+    DevBuf d_Jb, d_Kb, d_P, d_J, d_K;
+    // shell metadata on device (density-weighted screening)
+    DevBuf d_shell_f0, d_shell_nf;
+};
+
+}  // namespace
+
+struct fock_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string err;
+    DevBuf lut, tab64, tab32;
+    BoysTables bt{};
+    Prepared jk;
+    fock_status fail(fock_status st, const std::string &msg) {
+        err = msg;
+        return st;
+    }
+};
+
+#define HIPCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return ctx->fail(_e == hipErrorOutOfMemory ? FOCK_OOM : FOCK_HIP_ERROR,             \
+                             std::string(#expr) + ": " + hipGetErrorString(_e));                \
+    } while (0)
+
+#define STCHK(expr)                           \
+    do {                                      \
+        fock_status _s = (expr);              \
+        if (_s != FOCK_OK) return _s;         \
+    } while (0)
+
+namespace {
+
+// F_m(T) = exp(-T) sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in extended precision.
+long double boys_series(long double T, int m) {
+    long double term = 1.0L / (2.0L * m + 1.0L), sum = term;
+    for (int k = 1; k < 4000; ++k) {
+        term *= 2.0L * T / (2.0L * m + 2.0L * k + 1.0L);
+        sum += term;
+        if (term < 1e-22L * sum) break;
+    }
+    return expl(-T) * sum;
+}
+
+void make_boys_tables(std::vector<double> &t64, std::vector<float> &t32) {
+    t64.assign((size_t)BOYS_GRID_N * BOYS_ORDERS, 0.0);
+    t32.assign((size_t)BOYS_GRID_N * BOYS_ORDERS, 0.0f);
+    for (int i = 0; i < BOYS_GRID_N; ++i) {
+        long double T = (long double)i / 10.0L;
+        for (int m = 0; m < BOYS_ORDERS - 1; ++m) {
+            long double f = boys_series(T, m);
+            t64[(size_t)i * BOYS_ORDERS + m] = (double)f;
+            t32[(size_t)i * BOYS_ORDERS + m] = (float)f;
+        }
+        t64[(size_t)i * BOYS_ORDERS + BOYS_ORDERS - 1] = (double)expl(-T);
+        t32[(size_t)i * BOYS_ORDERS + BOYS_ORDERS - 1] = (float)expl(-T);
+    }
+}
+
+// Largest double x with sqrt(x) <= 1e-6: the reference keeps an integral iff
+// sqrt(D_xy D_zw) > 1e-6 (basis/src/electron_tensor.rs:134-149), i.e. iff D_xy D_zw > x.
+double ref_screen_threshold() {
+    double x = 1e-12;
+    while (std::sqrt(std::nextafter(x, 1.0)) <= 1e-6) x = std::nextafter(x, 1.0);
+    while (std::sqrt(x) > 1e-6) x = std::nextafter(x, 0.0);
+    return x;
+}
+
+int ncart_h(int L) { return (L + 1) * (L + 2) / 2; }
+
+int cart_h(int L, int c, int axis) {
+    int idx = 0;
+    for (int x = 0; x <= L; ++x)
+        for (int y = 0; y <= L; ++y) {
+            int z = L - x - y;
+            if (z < 0) continue;
+            if (idx == c) return axis == 0 ? x : (axis == 1 ? y : z);
+            ++idx;
+        }
+    return -1;
+}
+
+int pair_class_of(int La, int Lb) { return La == 0 ? 0 : (Lb == 0 ? 1 : 2); }
+
+// Group the flat function list into shells (basis_set/src/lib.rs:85-113 expands every shell
+// into ncart(L) consecutive functions with shared centre, exponents and coefficients).
+fock_status build_shells(fock_ctx *ctx, const fock_basis_desc *b, std::vector<Shell> &out) {
+    if (!b || b->n_functions <= 0 || !b->centers || !b->angular || !b->prim_offset || !b->exponents ||
+        !b->coefficients)
+        return ctx->fail(FOCK_INVALID_ARG, "basis descriptor is empty or has null arrays");
+    out.clear();
+    const int n = b->n_functions;
+    int f = 0;
+    while (f < n) {
+        const int *ang = b->angular + 3 * f;
+        if (ang[0] < 0 || ang[1] < 0 || ang[2] < 0) return ctx->fail(FOCK_INVALID_ARG, "negative angular momentum");
+        const int L = ang[0] + ang[1] + ang[2];
+        if (L > 1)
+            return ctx->fail(FOCK_UNSUPPORTED, "function " + std::to_string(f) + " has L=" + std::to_string(L) +
+                                                   "; kernels are built for s and p shells");
+        const int nf = ncart_h(L);
+        if (f + nf > n) return ctx->fail(FOCK_INVALID_ARG, "incomplete shell at function " + std::to_string(f));
+        const int K = b->prim_offset[f + 1] - b->prim_offset[f];
+        if (K <= 0) return ctx->fail(FOCK_INVALID_ARG, "function without primitives");
+        for (int c = 0; c < nf; ++c) {
+            const int g = f + c;
+            const int *ag = b->angular + 3 * g;
+            if (ag[0] != cart_h(L, c, 0) || ag[1] != cart_h(L, c, 1) || ag[2] != cart_h(L, c, 2))
+                return ctx->fail(FOCK_INVALID_ARG,
+                                 "function " + std::to_string(g) + " breaks the Cartesian component order of its shell");
+            if (b->prim_offset[g + 1] - b->prim_offset[g] != K)
+                return ctx->fail(FOCK_INVALID_ARG, "shell components with different contraction lengths");
+            // bitwise comparison: a NaN centre is carried through (the reference then returns None)
+            if (std::memcmp(b->centers + 3 * g, b->centers + 3 * f, 3 * sizeof(double)) != 0)
+                return ctx->fail(FOCK_INVALID_ARG, "shell components on different centres");
+            for (int k = 0; k < K; ++k) {
+                if (b->exponents[b->prim_offset[g] + k] != b->exponents[b->prim_offset[f] + k] ||
+                    b->coefficients[b->prim_offset[g] + k] != b->coefficients[b->prim_offset[f] + k])
+                    return ctx->fail(FOCK_INVALID_ARG, "shell components with different primitives");
+                if (!(b->exponents[b->prim_offset[g] + k] > 0.0))
+                    return ctx->fail(FOCK_INVALID_ARG, "non-positive exponent");
+            }
+        }
+        Shell s;
+        s.L = L;
+        s.f0 = f;
+        s.nf = nf;
+        s.K = K;
+        s.prim0 = b->prim_offset[f];
+        for (int x = 0; x < 3; ++x) s.A[x] = b->centers[3 * f + x];
+        out.push_back(s);
+        f += nf;
+    }
+    return FOCK_OK;
+}
+
+// Primitive pair data in the reference's arithmetic (contracted_gaussian.rs:270-292,
+// utils.rs:33-50): dist = B - A, q = ab/p, E^{00}_0 = exp(-q dist^2) per axis,
+// xa = -(q dist / a), xb = q dist / b.
+PrimPair<double> make_prim_pair(const fock_basis_desc *b, const Shell &sa, const Shell &sb, int i, int j) {
+    const double a = b->exponents[sa.prim0 + i], bb = b->exponents[sb.prim0 + j];
+    const double ca = b->coefficients[sa.prim0 + i], cb = b->coefficients[sb.prim0 + j];
+    const double p = a + bb, q = a * bb / p;
+    double dist[3], P[3], ex[3];
+    for (int x = 0; x < 3; ++x) {
+        dist[x] = sb.A[x] - sa.A[x];
+        P[x] = (a * sa.A[x] + bb * sb.A[x]) / p;
+        ex[x] = std::exp(-q * (dist[x] * dist[x]));
+    }
+    PrimPair<double> pp;
+    pp.p = p;
+    pp.Px = P[0];
+    pp.Py = P[1];
+    pp.Pz = P[2];
+    pp.Kc = ca * cb * (ex[0] * ex[1] * ex[2]) / p;
+    pp.xax = -(q * dist[0] / a);
+    pp.xay = -(q * dist[1] / a);
+    pp.xaz = -(q * dist[2] / a);
+    pp.xbx = q * dist[0] / bb;
+    pp.xby = q * dist[1] / bb;
+    pp.xbz = q * dist[2] / bb;
+    pp.inv2p = 1.0 / (2.0 * p);
+    return pp;
+}
+
+hipStream_t S(fock_ctx *ctx) { return ctx->stream; }
+
+}  // namespace
+
+// ====================================================================== context
+
+extern "C" const char *fock_version(void) { return "fock-mi355x 0.1 (gfx950)"; }
+
+extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
+    if (!out) return FOCK_INVALID_ARG;
+    *out = nullptr;
+    std::unique_ptr<fock_ctx> c(new fock_ctx());
+    fock_ctx *ctx = c.get();
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return FOCK_INVALID_ARG;
+    ctx->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+    ctx->stream = ctx->own;
+    if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+        return ctx->fail(FOCK_HIP_ERROR, "rocblas_create_handle failed");
+    rocblas_set_stream(ctx->blas, ctx->stream);
+    std::vector<double> t64;
+    std::vector<float> t32;
+    make_boys_tables(t64, t32);
+    HIPCHK(ctx->lut.ensure(sizeof(REF_LUT_HOST)));
+    HIPCHK(hipMemcpy(ctx->lut.p, REF_LUT_HOST, sizeof(REF_LUT_HOST), hipMemcpyHostToDevice));
+    HIPCHK(ctx->tab64.ensure(t64.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->tab64.p, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx->tab32.ensure(t32.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(ctx->tab32.p, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->bt.lut_ref = ctx->lut.as<double>();
+    ctx->bt.tab64 = ctx->tab64.as<double>();
+    ctx->bt.tab32 = ctx->tab32.as<float>();
+    *out = c.release();
+    return FOCK_OK;
+}
+
+extern "C" void fock_ctx_destroy(fock_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+}
+
+extern "C" const char *fock_ctx_last_error(const fock_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" fock_status fock_ctx_set_stream(fock_ctx *ctx, void *stream) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    ctx->stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->own;
+    rocblas_set_stream(ctx->blas, ctx->stream);
+    return FOCK_OK;
+}
+
+extern "C" void fock_scf_opts_default(fock_scf_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->max_iters = 100;
+    o->epsilon = 1e-6;
+    o->molecule_charge = 0;
+    o->integral_policy = FOCK_POLICY_REF_COMPAT;
+    o->screen = FOCK_SCREEN_REF_BF_PAIR;
+    o->schwarz_threshold = 1e-12;
+    o->precision = FOCK_PRECISION_FP64;
+    o->incremental = 0;
+}
+
+// ====================================================================== Boys
+
+extern "C" fock_status fock_boys(fock_ctx *ctx, const double *T, int64_t nT, int32_t n, int32_t policy,
+                                 double *out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (nT < 0 || (nT > 0 && (!T || !out))) return ctx->fail(FOCK_INVALID_ARG, "bad Boys arguments");
+    if (policy != 0 && policy != 1) return ctx->fail(FOCK_INVALID_ARG, "unknown policy");
+    if (n < 0) return ctx->fail(FOCK_INVALID_ARG, "negative Boys order");
+    if (policy == 0 && n > 10)
+        return ctx->fail(FOCK_BOYS_ORDER, "Boys-Function order " + std::to_string(n) +
+                                              " is higher than max tabulated order 10");
+    if (policy == 1 && n > BOYS_ORDERS - 1 - BOYS_TAYLOR64) return ctx->fail(FOCK_BOYS_ORDER, "Boys order above 11");
+    if (nT == 0) return FOCK_OK;
+    if (policy == 1)
+        for (int64_t i = 0; i < nT; ++i)
+            if (!(T[i] >= 0.0)) return ctx->fail(FOCK_INVALID_ARG, "accurate Boys needs T >= 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf dT, dO;
+    HIPCHK(dT.ensure(nT * sizeof(double)));
+    HIPCHK(dO.ensure(nT * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(dT.p, T, nT * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(launch_boys(dT.as<double>(), nT, n, policy, ctx->bt, dO.as<double>(), S(ctx)));
+    HIPCHK(hipMemcpyAsync(out, dO.p, nT * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+// ====================================================================== J/K preparation
+
+extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *basis, int32_t policy, int32_t screen,
+                                       double tau, int32_t precision) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (policy != 0 && policy != 1) return ctx->fail(FOCK_INVALID_ARG, "unknown integral policy");
+    if (screen < 0 || screen > 2) return ctx->fail(FOCK_INVALID_ARG, "unknown screen");
+    if (precision != 0 && precision != 1) return ctx->fail(FOCK_INVALID_ARG, "unknown precision");
+    if (precision == 1 && policy == 0)
+        return ctx->fail(FOCK_INVALID_ARG, "mixed precision is defined for the accurate policy only");
+    if (screen == FOCK_SCREEN_SCHWARZ && !(tau >= 0.0)) return ctx->fail(FOCK_INVALID_ARG, "bad Schwarz threshold");
+    HIPCHK(hipSetDevice(ctx->device));
+    Prepared &pr = ctx->jk;
+    pr.valid = false;
+    STCHK(build_shells(ctx, basis, pr.shells));
+    const int n = basis->n_functions;
+    const int ns = (int)pr.shells.size();
+    pr.n = n;
+    pr.policy = policy;
+    pr.screen = screen;
+    pr.precision = precision;
+    pr.tau = tau;
+    pr.dthr = ref_screen_threshold();
+
+    // every shell pair, oriented L(a) >= L(b) (same L: higher shell index first)
+    std::vector<PairHost> pairs;
+    std::vector<PrimPair<double>> prims;
+    for (int s1 = 0; s1 < ns; ++s1)
+        for (int s2 = 0; s2 <= s1; ++s2) {
+            int a = s1, b = s2;
+            if (pr.shells[b].L > pr.shells[a].L) std::swap(a, b);
+            PairHost ph;
+            ph.sa = a;
+            ph.sb = b;
+            ph.cls = pair_class_of(pr.shells[a].L, pr.shells[b].L);
+            ph.K = pr.shells[a].K * pr.shells[b].K;
+            ph.pp_start = (int)prims.size();
+            ph.fa = pr.shells[a].f0;
+            ph.fb = pr.shells[b].f0;
+            ph.creation = (int)pairs.size();
+            ph.dmax = 0.0;
+            for (int i = 0; i < pr.shells[a].K; ++i)
+                for (int j = 0; j < pr.shells[b].K; ++j) prims.push_back(make_prim_pair(basis, pr.shells[a], pr.shells[b], i, j));
+            pairs.push_back(ph);
+        }
+    const int npair = (int)pairs.size();
+    HIPCHK(pr.d_prims64.ensure(prims.size() * sizeof(PrimPair<double>)));
+    HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), prims.size() * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
+                          S(ctx)));
+    if (precision == 1) {
+        std::vector<PrimPair<float>> p32(prims.size());
+        for (size_t i = 0; i < prims.size(); ++i) {
+            const PrimPair<double> &d = prims[i];
+            p32[i] = PrimPair<float>{(float)d.p,   (float)d.Px,  (float)d.Py,  (float)d.Pz,
+                                     (float)d.Kc,  (float)d.xax, (float)d.xay, (float)d.xaz,
+                                     (float)d.xbx, (float)d.xby, (float)d.xbz, (float)d.inv2p};
+        }
+        HIPCHK(pr.d_prims32.ensure(p32.size() * sizeof(PrimPair<float>)));
+        HIPCHK(hipMemcpyAsync(pr.d_prims32.p, p32.data(), p32.size() * sizeof(PrimPair<float>), hipMemcpyHostToDevice,
+                              S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+    }
+
+    // diagonal (ab|ab): pairs grouped by class for the class-templated kernel
+    std::vector<int> by_class(npair);
+    std::iota(by_class.begin(), by_class.end(), 0);
+    std::stable_sort(by_class.begin(), by_class.end(), [&](int x, int y) { return pairs[x].cls < pairs[y].cls; });
+    std::vector<PairInfo> diag_info(npair);
+    for (int i = 0; i < npair; ++i) {
+        const PairHost &ph = pairs[by_class[i]];
+        diag_info[i] = PairInfo{ph.pp_start, ph.K, ph.fa, ph.fb};
+    }
+    HIPCHK(pr.d_pairs.ensure(std::max<size_t>(npair, 1) * sizeof(PairInfo)));
+    HIPCHK(hipMemcpyAsync(pr.d_pairs.p, diag_info.data(), npair * sizeof(PairInfo), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(pr.d_D.ensure((size_t)n * n * sizeof(double)));
+    HIPCHK(fill_f64((int64_t)n * n, 0.0, pr.d_D.as<double>(), S(ctx)));
+    {
+        int start = 0;
+        for (int c = 0; c < 3; ++c) {
+            int cnt = 0;
+            while (start + cnt < npair && pairs[by_class[start + cnt]].cls == c) ++cnt;
+            DiagArgs da{pr.d_pairs.as<PairInfo>(), start, cnt, n, pr.d_prims64.as<PrimPair<double>>(),
+                        pr.d_D.as<double>(), ctx->bt};
+            HIPCHK(launch_eri_diag(c, da, policy, S(ctx)));
+            start += cnt;
+        }
+    }
+    std::vector<double> D((size_t)n * n);
+    HIPCHK(hipMemcpyAsync(D.data(), pr.d_D.p, D.size() * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    for (PairHost &ph : pairs) {
+        const Shell &A = pr.shells[ph.sa], &B = pr.shells[ph.sb];
+        double m = 0.0;
+        for (int i = 0; i < A.nf; ++i)
+            for (int j = 0; j < B.nf; ++j) m = std::max(m, D[(size_t)(A.f0 + i) * n + (B.f0 + j)]);
+        ph.dmax = std::isnan(m) ? HUGE_VAL : m;  // NaN inputs: keep the pair, sort stays well defined
+    }
+
+    // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation)
+    std::vector<int> order(npair);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        const PairHost &a = pairs[x], &b = pairs[y];
+        if (a.cls != b.cls) return a.cls < b.cls;
+        if (a.K != b.K) return a.K < b.K;
+        if (a.dmax != b.dmax) return a.dmax > b.dmax;
+        return a.creation < b.creation;
+    });
+    std::vector<int> rank_in_class(npair);
+    {
+        std::vector<int> r(npair);
+        std::iota(r.begin(), r.end(), 0);
+        std::sort(r.begin(), r.end(), [&](int x, int y) {
+            const PairHost &a = pairs[x], &b = pairs[y];
+            if (a.cls != b.cls) return a.cls < b.cls;
+            if (a.dmax != b.dmax) return a.dmax > b.dmax;
+            return a.creation < b.creation;
+        });
+        for (int i = 0; i < npair; ++i) rank_in_class[r[i]] = i;
+    }
+    pr.pairs.resize(npair);
+    std::vector<PairInfo> info(npair);
+    std::vector<int> rank_sorted(npair);
+    for (int i = 0; i < npair; ++i) {
+        pr.pairs[i] = pairs[order[i]];
+        const PairHost &ph = pr.pairs[i];
+        info[i] = PairInfo{ph.pp_start, ph.K, ph.fa, ph.fb};
+        rank_sorted[i] = rank_in_class[order[i]];
+    }
+    HIPCHK(hipMemcpyAsync(pr.d_pairs.p, info.data(), npair * sizeof(PairInfo), hipMemcpyHostToDevice, S(ctx)));
+
+    // buckets: contiguous runs of equal (class, K)
+    struct Bucket {
+        int cls, K, begin, end;
+    };
+    std::vector<Bucket> buckets;
+    for (int i = 0; i < npair;) {
+        int j = i;
+        while (j < npair && pr.pairs[j].cls == pr.pairs[i].cls && pr.pairs[j].K == pr.pairs[i].K) ++j;
+        buckets.push_back(Bucket{pr.pairs[i].cls, pr.pairs[i].K, i, j});
+        i = j;
+    }
+    auto keep = [&](double di, double dj) -> bool {
+        if (screen == FOCK_SCREEN_NONE) return true;
+        if (screen == FOCK_SCREEN_SCHWARZ) return std::sqrt(di) * std::sqrt(dj) > tau;
+        return di * dj > pr.dthr;
+    };
+    std::vector<Task> tasks;
+    fock_jk_stats st{};
+    st.n_basis = n;
+    st.n_shells = ns;
+    st.n_shell_pairs = npair;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        pr.task_off[cls] = (int)tasks.size();
+        const int X = CLASS_BRA[cls], Y = CLASS_KET[cls];
+        std::vector<Task> ct;
+        for (int i = 0; i < npair; ++i) {
+            if (pr.pairs[i].cls != X) continue;
+            const double di = pr.pairs[i].dmax;
+            for (const Bucket &bk : buckets) {
+                if (bk.cls != Y) continue;
+                // hi: kets passing the screen form a prefix (dmax descending within the bucket)
+                int lo = bk.begin, hi = bk.end;
+                {
+                    int l = bk.begin, h = bk.end;
+                    while (l < h) {
+                        int m = (l + h) / 2;
+                        if (keep(di, pr.pairs[m].dmax)) l = m + 1;
+                        else h = m;
+                    }
+                    hi = l;
+                }
+                if (screen == FOCK_SCREEN_REF_BF_PAIR && X == Y && pr.pairs[i].K == bk.K && hi <= i && i < bk.end) {
+                    // the pair with itself holds the diagonal integrals the reference never screens
+                    hi = i + 1;
+                }
+                if (X == Y) {  // unique quartets: ket rank >= bra rank
+                    int l = bk.begin, h = hi;
+                    while (l < h) {
+                        int m = (l + h) / 2;
+                        if (rank_sorted[m] < rank_sorted[i]) l = m + 1;
+                        else h = m;
+                    }
+                    lo = l;
+                }
+                for (int s = lo; s < hi; s += 64) {
+                    const int cnt = std::min(64, hi - s);
+                    ct.push_back(Task{i, s, cnt, pr.pairs[i].K * bk.K});
+                    st.shell_quartets[cls] += cnt;
+                    st.primitive_quartets[cls] += (int64_t)cnt * pr.pairs[i].K * bk.K;
+                }
+            }
+        }
+        // most expensive tasks first (load balance); stable for determinism
+        std::stable_sort(ct.begin(), ct.end(), [](const Task &a, const Task &b) { return a.flags > b.flags; });
+        tasks.insert(tasks.end(), ct.begin(), ct.end());
+        st.tasks[cls] = (int64_t)ct.size();
+        st.integrals[cls] = st.shell_quartets[cls] * CLASS_NINT[cls];
+        st.algorithmic_flop[cls] = (double)st.primitive_quartets[cls] * F_PRIM[cls] + 12.0 * (double)st.integrals[cls];
+    }
+    pr.task_off[NUM_CLASSES] = (int)tasks.size();
+    if (screen == FOCK_SCREEN_REF_BF_PAIR) {
+        // basis-function pair products within 1e-12 of the reference's threshold (estimate)
+        std::vector<double> dv;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j <= i; ++j) dv.push_back(D[(size_t)i * n + j]);
+        std::sort(dv.begin(), dv.end());
+        int64_t near = 0;
+        for (double x : dv) {
+            if (x <= 0.0) continue;
+            const double lo = (1e-6 - 1e-12) * (1e-6 - 1e-12) / x, hi = (1e-6 + 1e-12) * (1e-6 + 1e-12) / x;
+            near += std::upper_bound(dv.begin(), dv.end(), hi) - std::lower_bound(dv.begin(), dv.end(), lo);
+        }
+        st.near_threshold = near;
+    }
+    pr.stats = st;
+    HIPCHK(pr.d_tasks.ensure(std::max<size_t>(tasks.size(), 1) * sizeof(Task)));
+    HIPCHK(hipMemcpyAsync(pr.d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(pr.d_Jb.ensure((size_t)n * n * sizeof(double)));
+    HIPCHK(pr.d_Kb.ensure((size_t)n * n * sizeof(double)));
+    std::vector<int> f0(ns), nf(ns);
+    for (int s = 0; s < ns; ++s) {
+        f0[s] = pr.shells[s].f0;
+        nf[s] = pr.shells[s].nf;
+    }
+    HIPCHK(pr.d_shell_f0.ensure(ns * sizeof(int)));
+    HIPCHK(pr.d_shell_nf.ensure(ns * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(pr.d_shell_f0.p, f0.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(pr.d_shell_nf.p, nf.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    pr.valid = true;
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
+    if (!ctx || !stats) return FOCK_INVALID_ARG;
+    if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
+    *stats = ctx->jk.stats;
+    return FOCK_OK;
+}
+
+namespace {
+
+// Enqueue the six class kernels accumulating into Jb/Kb (zeroed here).
+fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, int mode, double *dense) {
+    Prepared &pr = ctx->jk;
+    const int n = pr.n;
+    if (mode == MODE_JK) {
+        HIPCHK(fill_f64((int64_t)n * n, 0.0, Jb, S(ctx)));
+        HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
+    }
+    const bool fp32 = (pr.precision == 1) && mode == MODE_JK;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        JKArgs a{};
+        a.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
+        a.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
+        a.nbf = n;
+        a.pairs = pr.d_pairs.as<PairInfo>();
+        a.prims = fp32 ? pr.d_prims32.p : pr.d_prims64.p;
+        a.P = dP;
+        a.J = Jb;
+        a.K = Kb;
+        a.D = pr.d_D.as<double>();
+        a.dthr = pr.dthr;
+        a.bt = ctx->bt;
+        a.dense = dense;
+        HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
+    }
+    return FOCK_OK;
+}
+
+}  // namespace
+
+extern "C" fock_status fock_jk_build_device(fock_ctx *ctx, const double *dP, double *dJ, double *dK) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
+    if (!dP) return ctx->fail(FOCK_INVALID_ARG, "null density");
+    HIPCHK(hipSetDevice(ctx->device));
+    Prepared &pr = ctx->jk;
+    STCHK(enqueue_jk(ctx, dP, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr));
+    if (dJ) HIPCHK(symmetrize_add(pr.n, pr.d_Jb.as<double>(), dJ, S(ctx)));
+    if (dK) HIPCHK(symmetrize_add(pr.n, pr.d_Kb.as<double>(), dK, S(ctx)));
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_jk_build(fock_ctx *ctx, const double *P, double *J, double *K) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
+    if (!P) return ctx->fail(FOCK_INVALID_ARG, "null density");
+    HIPCHK(hipSetDevice(ctx->device));
+    Prepared &pr = ctx->jk;
+    const size_t bytes = (size_t)pr.n * pr.n * sizeof(double);
+    HIPCHK(pr.d_P.ensure(bytes));
+    HIPCHK(pr.d_J.ensure(bytes));
+    HIPCHK(pr.d_K.ensure(bytes));
+    HIPCHK(hipMemcpyAsync(pr.d_P.p, P, bytes, hipMemcpyHostToDevice, S(ctx)));
+    STCHK(fock_jk_build_device(ctx, pr.d_P.as<double>(), pr.d_J.as<double>(), pr.d_K.as<double>()));
+    if (J) HIPCHK(hipMemcpyAsync(J, pr.d_J.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    if (K) HIPCHK(hipMemcpyAsync(K, pr.d_K.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_eri_dense(fock_ctx *ctx, const fock_basis_desc *basis, int32_t policy, int32_t screen,
+                                      double tau, double *out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (!out) return ctx->fail(FOCK_INVALID_ARG, "null output");
+    STCHK(fock_jk_prepare(ctx, basis, policy, screen, tau, 0));
+    const size_t n = (size_t)ctx->jk.n;
+    const size_t len = n * n * n * n;
+    if (len > ((size_t)1 << 31)) return ctx->fail(FOCK_INVALID_ARG, "dense ERI hook is limited to n^4 <= 2^31");
+    DevBuf d;
+    HIPCHK(d.ensure(len * sizeof(double)));
+    HIPCHK(fill_f64((int64_t)len, 0.0, d.as<double>(), S(ctx)));
+    STCHK(enqueue_jk(ctx, nullptr, nullptr, nullptr, MODE_DENSE, d.as<double>()));
+    HIPCHK(hipMemcpyAsync(out, d.p, len * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+// ====================================================================== one-electron
+
+namespace {
+
+struct BasisDev {
+    DevBuf centers, angular, prim_off, exps, coefs, nuc_pos, nuc_q;
+    DevBuf shell_f0, pair_a, pair_b;
+    int npairs_cls[3] = {0, 0, 0};
+    int off_cls[3] = {0, 0, 0};
+};
+
+fock_status upload_one_electron(fock_ctx *ctx, const fock_basis_desc *b, const fock_nuclei_desc *nuc,
+                                const std::vector<Shell> &shells, BasisDev &bd) {
+    const int n = b->n_functions;
+    const int nprim = b->prim_offset[n];
+    HIPCHK(bd.centers.ensure(3 * n * sizeof(double)));
+    HIPCHK(bd.angular.ensure(3 * n * sizeof(int)));
+    HIPCHK(bd.prim_off.ensure((n + 1) * sizeof(int)));
+    HIPCHK(bd.exps.ensure(nprim * sizeof(double)));
+    HIPCHK(bd.coefs.ensure(nprim * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(bd.centers.p, b->centers, 3 * n * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(bd.angular.p, b->angular, 3 * n * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(bd.prim_off.p, b->prim_offset, (n + 1) * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(bd.exps.p, b->exponents, nprim * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(bd.coefs.p, b->coefficients, nprim * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    const int na = nuc ? nuc->n_atoms : 0;
+    HIPCHK(bd.nuc_pos.ensure(std::max(1, 3 * na) * sizeof(double)));
+    HIPCHK(bd.nuc_q.ensure(std::max(1, na) * sizeof(double)));
+    if (na > 0) {
+        HIPCHK(hipMemcpyAsync(bd.nuc_pos.p, nuc->positions, 3 * na * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(bd.nuc_q.p, nuc->charges, na * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    }
+    const int ns = (int)shells.size();
+    std::vector<int> f0(ns), pa, pb;
+    for (int s = 0; s < ns; ++s) f0[s] = shells[s].f0;
+    for (int c = 0; c < 3; ++c) {
+        bd.off_cls[c] = (int)pa.size();
+        for (int s1 = 0; s1 < ns; ++s1)
+            for (int s2 = 0; s2 <= s1; ++s2) {
+                int a = s1, bb = s2;
+                if (shells[bb].L > shells[a].L) std::swap(a, bb);
+                if (pair_class_of(shells[a].L, shells[bb].L) != c) continue;
+                pa.push_back(a);
+                pb.push_back(bb);
+            }
+        bd.npairs_cls[c] = (int)pa.size() - bd.off_cls[c];
+    }
+    HIPCHK(bd.shell_f0.ensure(ns * sizeof(int)));
+    HIPCHK(bd.pair_a.ensure(std::max<size_t>(pa.size(), 1) * sizeof(int)));
+    HIPCHK(bd.pair_b.ensure(std::max<size_t>(pb.size(), 1) * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(bd.shell_f0.p, f0.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(bd.pair_a.p, pa.data(), pa.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(bd.pair_b.p, pb.data(), pb.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    return FOCK_OK;
+}
+
+fock_status enqueue_one_electron(fock_ctx *ctx, const fock_basis_desc *b, const fock_nuclei_desc *nuc,
+                                 BasisDev &bd, int policy, double *dS, double *dT, double *dV) {
+    OneElArgs a{};
+    a.nbf = b->n_functions;
+    a.natoms = nuc ? nuc->n_atoms : 0;
+    a.centers = bd.centers.as<double>();
+    a.angular = bd.angular.as<int>();
+    a.prim_off = bd.prim_off.as<int>();
+    a.exps = bd.exps.as<double>();
+    a.coefs = bd.coefs.as<double>();
+    a.nuc_pos = bd.nuc_pos.as<double>();
+    a.nuc_charge = bd.nuc_q.as<double>();
+    a.S = dS;
+    a.T = dT;
+    a.V = dV;
+    a.bt = ctx->bt;
+    for (int c = 0; c < 3; ++c)
+        HIPCHK(launch_one_electron_class(c, a, bd.shell_f0.as<int>(), bd.pair_a.as<int>() + bd.off_cls[c],
+                                         bd.pair_b.as<int>() + bd.off_cls[c], bd.npairs_cls[c], policy, S(ctx)));
+    return FOCK_OK;
+}
+
+}  // namespace
+
+extern "C" fock_status fock_one_electron(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
+                                         int32_t policy, double *Sm, double *Tm, double *Vm) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (policy != 0 && policy != 1) return ctx->fail(FOCK_INVALID_ARG, "unknown integral policy");
+    if (nuclei && nuclei->n_atoms > 0 && (!nuclei->positions || !nuclei->charges))
+        return ctx->fail(FOCK_INVALID_ARG, "nuclei with null arrays");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<Shell> shells;
+    STCHK(build_shells(ctx, basis, shells));
+    BasisDev bd;
+    STCHK(upload_one_electron(ctx, basis, nuclei, shells, bd));
+    const size_t n = basis->n_functions, bytes = n * n * sizeof(double);
+    DevBuf dS, dT, dV;
+    HIPCHK(dS.ensure(bytes));
+    HIPCHK(dT.ensure(bytes));
+    HIPCHK(dV.ensure(bytes));
+    STCHK(enqueue_one_electron(ctx, basis, nuclei, bd, policy, dS.as<double>(), dT.as<double>(), dV.as<double>()));
+    if (Sm) HIPCHK(hipMemcpyAsync(Sm, dS.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    if (Tm) HIPCHK(hipMemcpyAsync(Tm, dT.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    if (Vm) HIPCHK(hipMemcpyAsync(Vm, dV.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+// ====================================================================== SCF
+
+namespace {
+
+// Symmetric eigendecomposition on the GPU (rocSOLVER dsyevd). A (n x n, symmetric) is
+// overwritten by R with R[k][i] = i-th component of eigenvector k (row-major), w ascending.
+fock_status eig_sym(fock_ctx *ctx, int n, double *A, double *w, double *work, int *info) {
+    rocblas_status st = rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, work, info);
+    if (st != rocblas_status_success)
+        return ctx->fail(FOCK_HIP_ERROR, "rocsolver_dsyevd failed with status " + std::to_string((int)st));
+    return FOCK_OK;
+}
+
+bool is_normal(double x) { return std::isnormal(x); }
+
+// DIIS coefficients (scf/src/utils.rs:102-138): B c = (0, ..., 0, -1) with B_ij = <e_i, e_j>,
+// bordered by -1. Solved by Gaussian elimination with partial pivoting; false when a pivot
+// is exactly zero (the reference falls back to the last Fock matrix when try_inverse fails).
+bool diis_solve(int m, const std::vector<double> &Bm, std::vector<double> &c) {
+    const int N = m + 1;
+    std::vector<double> A((size_t)N * (N + 1), 0.0);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) A[(size_t)i * (N + 1) + j] = Bm[(size_t)i * m + j];
+    for (int i = 0; i < m; ++i) {
+        A[(size_t)i * (N + 1) + m] = -1.0;
+        A[(size_t)m * (N + 1) + i] = -1.0;
+    }
+    A[(size_t)m * (N + 1) + m] = 0.0;
+    A[(size_t)m * (N + 1) + N] = -1.0;
+    for (int col = 0; col < N; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < N; ++r)
+            if (std::fabs(A[(size_t)r * (N + 1) + col]) > std::fabs(A[(size_t)piv * (N + 1) + col])) piv = r;
+        if (A[(size_t)piv * (N + 1) + col] == 0.0) return false;
+        if (piv != col)
+            for (int k = 0; k <= N; ++k) std::swap(A[(size_t)piv * (N + 1) + k], A[(size_t)col * (N + 1) + k]);
+        for (int r = 0; r < N; ++r) {
+            if (r == col) continue;
+            const double f = A[(size_t)r * (N + 1) + col] / A[(size_t)col * (N + 1) + col];
+            if (f == 0.0) continue;
+            for (int k = col; k <= N; ++k) A[(size_t)r * (N + 1) + k] -= f * A[(size_t)col * (N + 1) + k];
+        }
+    }
+    c.resize(m);
+    for (int i = 0; i < m; ++i) {
+        c[i] = A[(size_t)i * (N + 1) + N] / A[(size_t)i * (N + 1) + i];
+        if (!std::isfinite(c[i])) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" fock_status fock_gemm(fock_ctx *ctx, int32_t transA, int32_t transB, int32_t m, int32_t n, int32_t k,
+                                 double alpha, const double *A, int32_t lda, const double *B, int32_t ldb, double beta,
+                                 double *C, int32_t ldc) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (m < 0 || n < 0 || k < 0 || !A || !B || !C) return ctx->fail(FOCK_INVALID_ARG, "bad GEMM arguments");
+    const int64_t arows = transA ? k : m, acols = transA ? m : k;
+    const int64_t brows = transB ? n : k, bcols = transB ? k : n;
+    if (lda < acols || ldb < bcols || ldc < n) return ctx->fail(FOCK_INVALID_ARG, "leading dimension too small");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf dA, dB, dC;
+    HIPCHK(dA.ensure(std::max<int64_t>(arows * lda, 1) * sizeof(double)));
+    HIPCHK(dB.ensure(std::max<int64_t>(brows * ldb, 1) * sizeof(double)));
+    HIPCHK(dC.ensure(std::max<int64_t>((int64_t)m * ldc, 1) * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(dA.p, A, arows * lda * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dB.p, B, brows * ldb * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dC.p, C, (int64_t)m * ldc * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(gemm_f64(transA != 0, transB != 0, m, n, k, alpha, dA.as<double>(), lda, dB.as<double>(), ldb, beta,
+                    dC.as<double>(), ldc, S(ctx)));
+    HIPCHK(hipMemcpyAsync(C, dC.p, (int64_t)m * ldc * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_sorted_eigs(fock_ctx *ctx, int32_t n, const double *M, double *vectors, double *values) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (n <= 0 || !M || !vectors || !values) return ctx->fail(FOCK_INVALID_ARG, "bad eigen arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)n * n * sizeof(double);
+    DevBuf dA, dW, dE, dI;
+    HIPCHK(dA.ensure(bytes));
+    HIPCHK(dW.ensure(n * sizeof(double)));
+    HIPCHK(dE.ensure(n * sizeof(double)));
+    HIPCHK(dI.ensure(sizeof(int)));
+    HIPCHK(hipMemcpyAsync(dA.p, M, bytes, hipMemcpyHostToDevice, S(ctx)));
+    STCHK(eig_sym(ctx, n, dA.as<double>(), dW.as<double>(), dE.as<double>(), dI.as<int>()));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(vectors, dA.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipMemcpyAsync(values, dW.p, n * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipMemcpyAsync(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    if (info != 0) return ctx->fail(FOCK_HIP_ERROR, "dsyevd did not converge (info " + std::to_string(info) + ")");
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const double *errors, const double *focks,
+                                 double *out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (m <= 0 || m > 64 || len <= 0 || !errors || !focks || !out) return ctx->fail(FOCK_INVALID_ARG, "bad DIIS arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf dE, dF, dDots, dC, dSl, dO;
+    HIPCHK(dE.ensure(m * len * sizeof(double)));
+    HIPCHK(dF.ensure(m * len * sizeof(double)));
+    HIPCHK(dDots.ensure(m * sizeof(double)));
+    HIPCHK(dC.ensure(m * sizeof(double)));
+    HIPCHK(dSl.ensure(m * sizeof(int)));
+    HIPCHK(dO.ensure(len * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(dE.p, errors, m * len * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dF.p, focks, m * len * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    std::vector<double> Bm((size_t)m * m), row(m);
+    for (int i = 0; i < m; ++i) {
+        HIPCHK(frob_dots(len, dE.as<double>() + i * len, dE.as<double>(), len, m, dDots.as<double>(), S(ctx)));
+        HIPCHK(hipMemcpyAsync(row.data(), dDots.p, m * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+        for (int j = 0; j < m; ++j) Bm[(size_t)i * m + j] = row[j];
+    }
+    std::vector<double> c;
+    if (!diis_solve(m, Bm, c)) return ctx->fail(FOCK_INVALID_ARG, "DIIS matrix is singular");
+    std::vector<int> sl(m);
+    std::iota(sl.begin(), sl.end(), 0);
+    HIPCHK(hipMemcpyAsync(dC.p, c.data(), m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dSl.p, sl.data(), m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(lincomb(len, dF.as<double>(), len, dSl.as<int>(), dC.as<double>(), m, dO.as<double>(), S(ctx)));
+    HIPCHK(hipMemcpyAsync(out, dO.p, len * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
+                                    const fock_scf_opts *opts_in, fock_scf_out *out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (!basis || !nuclei || !out) return ctx->fail(FOCK_INVALID_ARG, "null basis, nuclei or output");
+    fock_scf_opts opts;
+    if (opts_in) opts = *opts_in;
+    else fock_scf_opts_default(&opts);
+    if (opts.max_iters < 0) return ctx->fail(FOCK_INVALID_ARG, "negative max_iters");
+    if (nuclei->n_atoms <= 0 || !nuclei->positions || !nuclei->charges)
+        return ctx->fail(FOCK_INVALID_ARG, "no nuclei");
+    if (opts.incremental && opts.precision != FOCK_PRECISION_MIXED)
+        return ctx->fail(FOCK_INVALID_ARG, "incremental builds are part of the mixed-precision form");
+    const auto t_start = std::chrono::steady_clock::now();
+    const bool debug = std::getenv("FOCK_DEBUG") != nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = basis->n_functions;
+    const int64_t nn = (int64_t)n * n;
+    const size_t bytes = (size_t)nn * sizeof(double);
+
+    // electrons and nuclear repulsion (scf/src/lib.rs:91-116)
+    double zsum = 0.0;
+    for (int a = 0; a < nuclei->n_atoms; ++a) zsum += nuclei->charges[a];
+    const int n_electrons = (int)std::llround(zsum) - opts.molecule_charge;
+    if (n_electrons < 0 || n_electrons / 2 > n) return ctx->fail(FOCK_INVALID_ARG, "bad electron count");
+    const int nocc = n_electrons / 2;
+    double e_nuc = 0.0;
+    for (int i = 0; i < nuclei->n_atoms; ++i)
+        for (int j = i + 1; j < nuclei->n_atoms; ++j) {
+            const double *A = nuclei->positions + 3 * i, *B = nuclei->positions + 3 * j;
+            const double d[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+            e_nuc += nuclei->charges[i] * nuclei->charges[j] / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        }
+
+    // direct J/K state (screen lists, pair data)
+    const bool mixed = opts.precision == FOCK_PRECISION_MIXED;
+    STCHK(fock_jk_prepare(ctx, basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0));
+    Prepared &pr = ctx->jk;
+
+    // device matrices
+    DevBuf dS, dT, dV, dH, dX, dU, dTmp, dTmp2, dP, dG, dF, dFx, dC, dW, dE, dWork, dInfo, dErrs, dFocks, dDots,
+        dScal, dCoef, dSlots;
+    for (DevBuf *b : {&dS, &dT, &dV, &dH, &dX, &dU, &dTmp, &dTmp2, &dP, &dG, &dF, &dFx, &dC}) HIPCHK(b->ensure(bytes));
+    HIPCHK(dW.ensure(n * sizeof(double)));
+    HIPCHK(dE.ensure(n * sizeof(double)));
+    HIPCHK(dWork.ensure(n * sizeof(double)));
+    HIPCHK(dInfo.ensure(sizeof(int)));
+    HIPCHK(dErrs.ensure(16 * bytes));
+    HIPCHK(dFocks.ensure(16 * bytes));
+    HIPCHK(dDots.ensure(16 * sizeof(double)));
+    HIPCHK(dScal.ensure(4 * sizeof(double)));
+    HIPCHK(dCoef.ensure(16 * sizeof(double)));
+    HIPCHK(dSlots.ensure(16 * sizeof(int)));
+    HIPCHK(fill_f64(16 * nn, 0.0, dErrs.as<double>(), S(ctx)));
+    double *S_ = dS.as<double>(), *H_ = dH.as<double>(), *X_ = dX.as<double>(), *U_ = dU.as<double>();
+    double *T1 = dTmp.as<double>(), *T2 = dTmp2.as<double>(), *P_ = dP.as<double>(), *G_ = dG.as<double>();
+    double *F_ = dF.as<double>(), *Fx_ = dFx.as<double>(), *C_ = dC.as<double>();
+    double *W_ = dW.as<double>(), *scal = dScal.as<double>();
+
+    // S, T, V on the GPU (scf/src/lib.rs:118-129), H = T + V (:138)
+    {
+        BasisDev bd;
+        STCHK(upload_one_electron(ctx, basis, nuclei, pr.shells, bd));
+        STCHK(enqueue_one_electron(ctx, basis, nuclei, bd, opts.integral_policy, S_, dT.as<double>(), dV.as<double>()));
+        HIPCHK(add_f64(nn, dT.as<double>(), dV.as<double>(), H_, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+    }
+
+    // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146)
+    HIPCHK(copy_f64(nn, S_, T1, S(ctx)));
+    STCHK(eig_sym(ctx, n, T1, W_, dWork.as<double>(), dInfo.as<int>()));  // T1 = R (rows = eigenvectors)
+    HIPCHK(transpose_f64(n, T1, U_, S(ctx)));                               // U: columns = eigenvectors
+    HIPCHK(gemm_f64(false, false, n, n, n, 1.0, S_, n, U_, n, 0.0, T2, n, S(ctx)));   // S U
+    HIPCHK(gemm_f64(true, false, n, n, n, 1.0, U_, n, T2, n, 0.0, T1, n, S(ctx)));    // U^T S U
+    HIPCHK(scale_columns(n, U_, T1, n, T2, S(ctx)));                                   // U diag
+    HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U_, n, 0.0, X_, n, S(ctx)));    // (U diag) U^T
+
+    // solve F' = X^T M X; C = X C'; P = 2 C_occ C_occ^T; eigenvalues in W_
+    auto diagonalize = [&](const double *M) -> fock_status {
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, M, n, X_, n, 0.0, T1, n, S(ctx)));  // M X
+        HIPCHK(gemm_f64(true, false, n, n, n, 1.0, X_, n, T1, n, 0.0, T2, n, S(ctx)));  // X^T M X
+        STCHK(eig_sym(ctx, n, T2, W_, dWork.as<double>(), dInfo.as<int>()));          // T2 = R'
+        HIPCHK(gemm_f64(false, true, n, n, n, 1.0, X_, n, T2, n, 0.0, C_, n, S(ctx)));  // C = X R'^T
+        HIPCHK(density_from_coeffs(n, nocc, C_, P_, S(ctx)));
+        return FOCK_OK;
+    };
+
+    // GWH guess (scf/src/lib.rs:150-162)
+    HIPCHK(gwh_guess(n, S_, H_, F_, S(ctx)));
+    STCHK(diagonalize(F_));
+
+    std::deque<int> slots;        // ring slots of the stored (error, Fock) pairs, oldest first
+    std::deque<std::vector<double>> dots;  // dots[i][j] = <e_i, e_j> for stored i, j (same order)
+    int next_slot = 0;
+    hipEvent_t ev0, ev1;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } evg{ev0, ev1};
+    out->history_len = 0;
+    fock_status result = FOCK_NOT_CONVERGED;
+    for (int iter = 0; iter <= opts.max_iters; ++iter) {
+        // G(P), F = H + G (scf/src/lib.rs:177-187)
+        HIPCHK(hipEventRecord(ev0, S(ctx)));
+        STCHK(enqueue_jk(ctx, P_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr));
+        HIPCHK(hipEventRecord(ev1, S(ctx)));
+        HIPCHK(fock_from_jk(n, H_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), G_, F_, 0.5, S(ctx)));
+        // DIIS error e = F P S - S P F = (F P S) - (F P S)^T (F, P, S symmetric) (:189-192)
+        const int slot = next_slot;
+        next_slot = (next_slot + 1) % 16;
+        double *e_slot = dErrs.as<double>() + (int64_t)slot * nn;
+        double *f_slot = dFocks.as<double>() + (int64_t)slot * nn;
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, F_, n, P_, n, 0.0, T1, n, S(ctx)));   // F P
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, T1, n, S_, n, 0.0, T2, n, S(ctx)));   // F P S
+        HIPCHK(fill_f64(1, 0.0, scal, S(ctx)));
+        HIPCHK(antisym_maxabs(n, T2, e_slot, scal, S(ctx)));
+        HIPCHK(copy_f64(nn, F_, f_slot, S(ctx)));
+        // window of 16 (:194-200)
+        if ((int)slots.size() == 16) {
+            slots.pop_front();
+            dots.pop_front();
+            for (auto &row : dots) row.erase(row.begin());
+        }
+        slots.push_back(slot);
+        const int m = (int)slots.size();
+        // <e_new, e_i> for every stored i (ring slots in storage order)
+        std::vector<double> newdots(m);
+        {
+            // dots against all 16 ring slots, pick the stored ones
+            HIPCHK(frob_dots(nn, e_slot, dErrs.as<double>(), nn, 16, dDots.as<double>(), S(ctx)));
+            double hd[16 + 1];
+            HIPCHK(hipMemcpyAsync(hd, dDots.p, 16 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            HIPCHK(hipMemcpyAsync(hd + 16, scal, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            HIPCHK(hipStreamSynchronize(S(ctx)));
+            for (int i = 0; i < m; ++i) newdots[i] = hd[slots[i]];
+            out->last_error = hd[16];
+        }
+        const double err = out->last_error;
+        for (int i = 0; i < m - 1; ++i) dots[i].push_back(newdots[i]);
+        dots.push_back(newdots);
+        // extrapolate from 5 stored matrices on (:202-207)
+        const double *Fuse = f_slot;
+        if (m >= 5) {
+            std::vector<double> Bm((size_t)m * m), c;
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) Bm[(size_t)i * m + j] = dots[i][j];
+            if (diis_solve(m, Bm, c)) {
+                std::vector<int> sl(slots.begin(), slots.end());
+                HIPCHK(hipMemcpyAsync(dCoef.p, c.data(), m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+                HIPCHK(hipMemcpyAsync(dSlots.p, sl.data(), m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+                HIPCHK(lincomb(nn, dFocks.as<double>(), nn, dSlots.as<int>(), dCoef.as<double>(), m, Fx_, S(ctx)));
+                HIPCHK(hipStreamSynchronize(S(ctx)));  // c, sl are host temporaries
+                Fuse = Fx_;
+            }
+        }
+        // F' = X^T F X, eigen, C = X C', P_new (:210-216)
+        STCHK(diagonalize(Fuse));
+        // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
+        HIPCHK(energy_trace(n, P_, H_, G_, scal + 1, S(ctx)));
+        double e_elec = 0.0;
+        HIPCHK(hipMemcpyAsync(&e_elec, scal + 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        if (out->history && out->history_len < out->history_capacity) {
+            fock_scf_iter &h = out->history[out->history_len++];
+            h.total_energy = e_elec + e_nuc;
+            h.error = err;
+            h.fock_build_ms = ms;
+            int64_t q = 0;
+            for (int c = 0; c < NUM_CLASSES; ++c) q += pr.stats.shell_quartets[c];
+            h.quartets = q;
+        }
+        if (debug)
+            std::fprintf(stderr, "[fock] iter %d err %.6e E %.12f build %.3f ms\n", iter, err, e_elec + e_nuc, ms);
+        out->electronic_energy = e_elec;
+        out->total_energy = e_elec + e_nuc;
+        out->iterations = iter;
+        if (err < opts.epsilon || iter == opts.max_iters) {
+            result = (err < opts.epsilon) ? FOCK_OK : FOCK_NOT_CONVERGED;
+            break;
+        } else if (!is_normal(err)) {
+            result = FOCK_DIVERGED;
+            break;
+        }
+    }
+    out->n_electrons = n_electrons;
+    out->n_basis = n;
+    if (result == FOCK_DIVERGED) {
+        ctx->err = "SCF diverged: max|FDS - SDF| is not a normal number";
+        return result;
+    }
+    // outputs, column-major (nalgebra layout): C_cm[i + k n] = C[i][k] -> transpose of row-major
+    if (out->coefficients) {
+        HIPCHK(transpose_f64(n, C_, T1, S(ctx)));
+        HIPCHK(hipMemcpyAsync(out->coefficients, T1, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    }
+    if (out->density) HIPCHK(hipMemcpyAsync(out->density, P_, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    if (out->orbital_energies) HIPCHK(hipMemcpyAsync(out->orbital_energies, W_, n * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    out->scf_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (result == FOCK_NOT_CONVERGED) ctx->err = "iteration limit reached";
+    return result;
+}

@@ -1,0 +1,102 @@
+// kernels.hpp -- launch interface between the host driver and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fock_device.hpp"
+
+namespace fock {
+
+enum { MODE_JK = 0, MODE_DENSE = 1 };
+
+// Quartet classes (bra pair class, ket pair class) with L(bra) >= L(ket):
+// 0 (ss|ss), 1 (ps|ss), 2 (ps|ps), 3 (pp|ss), 4 (pp|ps), 5 (pp|pp)
+constexpr int NUM_CLASSES = 6;
+constexpr int CLASS_BRA[NUM_CLASSES] = {0, 1, 1, 2, 2, 2};  // pair classes: 0 ss, 1 ps, 2 pp
+constexpr int CLASS_KET[NUM_CLASSES] = {0, 0, 1, 0, 1, 2};
+
+struct JKArgs {
+    const Task *tasks;
+    int ntasks;
+    int nbf;
+    const PairInfo *pairs;
+    const void *prims;  // PrimPair<double> or PrimPair<float>
+    const double *P;
+    double *J;
+    double *K;
+    const double *D;  // (xy|xy) of every basis-function pair (reference screen)
+    double dthr;
+    BoysTables bt;
+    double *dense;  // MODE_DENSE output, n^4
+};
+
+struct DiagArgs {
+    const PairInfo *pairs;
+    int first, npairs, nbf;
+    const PrimPair<double> *prims;
+    double *D;
+    BoysTables bt;
+};
+
+hipError_t launch_eri_tasks(int cls, int mode, const JKArgs &a, int policy, bool refscreen, bool fp32,
+                            hipStream_t s);
+hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);
+
+// one_electron.hip
+struct OneElArgs {
+    int nbf, natoms;
+    const double *centers;
+    const int *angular;
+    const int *prim_off;
+    const double *exps;
+    const double *coefs;
+    const double *nuc_pos;
+    const double *nuc_charge;
+    double *S, *T, *V;
+    BoysTables bt;
+};
+hipError_t launch_one_electron_class(int pair_class, const OneElArgs &a, const int *shell_f0, const int *pair_a,
+                                     const int *pair_b, int npairs, int policy, hipStream_t s);
+
+// boys_kernel.hip
+hipError_t launch_boys(const double *T, int64_t nT, int n, int policy, const BoysTables &bt, double *out,
+                       hipStream_t s);
+
+// dense.hip: FP64 MFMA GEMM and small dense helpers (row-major n x n unless stated)
+// C = alpha * op(A) op(B) + beta * C, op = transpose if flag set; A: m x k, B: k x n, C: m x n
+hipError_t gemm_f64(bool transA, bool transB, int m, int n, int k, double alpha, const double *A, int lda,
+                    const double *B, int ldb, double beta, double *C, int ldc, hipStream_t s);
+// out = x + x^T (n x n)
+hipError_t symmetrize_add(int n, const double *x, double *out, hipStream_t s);
+// F = H + J - 0.5 K (with J, K the non-symmetric accumulation buffers, symmetrised here)
+hipError_t fock_from_jk(int n, const double *H, const double *Jb, const double *Kb, double *G, double *F,
+                        double scaleG, hipStream_t s);
+// e = M - M^T, and reduce max|e| into *maxabs (device scalar, zero-initialised by the kernel)
+hipError_t antisym_maxabs(int n, const double *M, double *e, double *maxabs, hipStream_t s);
+// dots[i] = <a, b_i> (Frobenius) for i < count, b_i = base + i * stride
+hipError_t frob_dots(int64_t len, const double *a, const double *base, int64_t stride, int count, double *dots,
+                     hipStream_t s);
+// out = sum_i c[i] * (base + slot[i] * stride), c and slots on device
+hipError_t lincomb(int64_t len, const double *base, int64_t stride, const int *slots, const double *c, int count,
+                   double *out, hipStream_t s);
+// out[i][j] = 2 * sum_{k<nocc} C[i][k] C[j][k]  (C row-major n x n)
+hipError_t density_from_coeffs(int n, int nocc, const double *C, double *P, hipStream_t s);
+// *out = 0.5 * sum_ij P_ij (2 H_ij + G_ij)
+hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *out, hipStream_t s);
+// x = a - b elementwise (len)
+hipError_t sub_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s);
+hipError_t add_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s);
+hipError_t transpose_f64(int n, const double *in, double *out, hipStream_t s);
+hipError_t fill_f64(int64_t len, double v, double *out, hipStream_t s);
+hipError_t copy_f64(int64_t len, const double *in, double *out, hipStream_t s);
+// GWH: out_ij = 0.875 S_ij (H_ii + H_jj)
+hipError_t gwh_guess(int n, const double *S, const double *H, double *out, hipStream_t s);
+// X = U diag(1/sqrt(D_ii)) U^T with D = U^T S U: scale columns of U by dinv (device vector)
+hipError_t scale_columns(int n, const double *U, const double *dvec_diag_source, int ld_diag, double *out,
+                         hipStream_t s);
+// per shell-pair max |P| blocks for density-weighted screening
+hipError_t shell_block_maxabs(int nshell, const int *shell_f0, const int *shell_nf, int n, const double *P,
+                              float *out, hipStream_t s);
+
+}  // namespace fock

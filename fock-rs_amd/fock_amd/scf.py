@@ -1,0 +1,124 @@
+"""Python mirror of the reference's `scf` crate API over the C ABI.
+
+    SelfConsistentField.try_scf(atoms, max_iters, epsilon, molecule_charge)
+        -> Optional[HartreeFockResult]                      (scf/src/lib.rs:48-78)
+    HartreeFockResult fields                                (scf/src/lib.rs:27-37)
+    MolecularOrbitals / MolecularOrbital.evaluate           (scf/src/molecular_orbitals.rs)
+
+Same argument meaning and outcome rules as the reference: the result is returned
+when max|FDS - SDF| < epsilon or at the last iteration (converged or not), and None
+when the error stops being a normal float. The work runs in libfock.so on the GPU.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Iterable, List, Optional
+
+import numpy as np
+
+from . import abi
+from .molecule import Atom, ContractedGaussian, flatten
+
+_CTX = None
+
+
+def context() -> abi.Context:
+    global _CTX
+    if _CTX is None:
+        _CTX = abi.Context(0)
+    return _CTX
+
+
+def evaluate_function(cg: ContractedGaussian, at) -> float:
+    """basis/src/contracted_gaussian.rs:320-331: sum_k c_k (A-r)^{ijk} exp(-a |A-r|^2)."""
+    diff = cg.position - np.asarray(at, dtype=np.float64)
+    d2 = float(diff @ diff)
+    acc = 0.0
+    for p in cg.primitives:
+        i, j, k = p.angular
+        acc += p.coefficient * diff[0] ** i * diff[1] ** j * diff[2] ** k * math.exp(-p.exponent * d2)
+    return acc
+
+
+class MolecularOrbital:
+    """scf/src/molecular_orbitals.rs:10-44."""
+
+    def __init__(self, basis_functions: List[ContractedGaussian], coefficients: np.ndarray):
+        self.basis_functions = basis_functions
+        self.coefficients = coefficients
+
+    def evaluate(self, at) -> float:
+        return float(sum(c * evaluate_function(b, at) for c, b in zip(self.coefficients, self.basis_functions)))
+
+
+class MolecularOrbitals:
+    """scf/src/molecular_orbitals.rs:46-99: keeps coefficients with |c| > 0.05 per orbital."""
+    MIN_COEFFICIENT_MAGNITUDE = 0.05
+
+    def __init__(self, basis_functions: List[ContractedGaussian], coeff_matrix: np.ndarray):
+        self.molecular_orbitals = []
+        for k in range(coeff_matrix.shape[1]):
+            col = coeff_matrix[:, k]
+            idx = [i for i in range(len(col)) if abs(col[i]) > self.MIN_COEFFICIENT_MAGNITUDE]
+            self.molecular_orbitals.append(MolecularOrbital([basis_functions[i] for i in idx],
+                                                            np.array([col[i] for i in idx])))
+
+    def __getitem__(self, index: int) -> MolecularOrbital:
+        return self.molecular_orbitals[index]
+
+    def __len__(self) -> int:
+        return len(self.molecular_orbitals)
+
+
+@dataclass
+class HartreeFockResult:
+    """scf/src/lib.rs:27-37."""
+    orbitals: MolecularOrbitals
+    orbital_energies: np.ndarray
+    density_matrix: np.ndarray
+    coefficient_matrix: np.ndarray
+    electronic_energy: float
+    total_energy: float
+    iterations: int
+    n_electrons: int
+    n_basis: int
+    history: list
+    scf_wall_ms: float
+    converged: bool
+
+
+def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_charge: int = 0,
+            policy: int = abi.POLICY_REF_COMPAT, screen: int = abi.SCREEN_REF_BF_PAIR,
+            schwarz_threshold: float = 1e-12, precision: int = abi.PRECISION_FP64,
+            incremental: bool = False, ctx: Optional[abi.Context] = None) -> Optional[HartreeFockResult]:
+    """SelfConsistentField::try_scf (scf/src/lib.rs:73-258) on the GPU.
+
+    Defaults reproduce the reference's own arithmetic (REF_COMPAT integrals, its 1e-6
+    basis-function-pair screen, FP64)."""
+    atoms = list(atoms)
+    fb, fn = flatten(atoms)
+    opts = abi.default_opts(max_iters=max_iters, epsilon=epsilon, molecule_charge=molecule_charge,
+                            integral_policy=policy, screen=screen, schwarz_threshold=schwarz_threshold,
+                            precision=precision, incremental=1 if incremental else 0)
+    ctx = ctx or context()
+    st, r = ctx.scf_run(fb, fn, opts)
+    if st == abi.FOCK_DIVERGED:
+        return None
+    basis = [cg for a in atoms for cg in a.basis]
+    return HartreeFockResult(
+        orbitals=MolecularOrbitals(basis, r["coefficients"]),
+        orbital_energies=r["orbital_energies"], density_matrix=r["density"],
+        coefficient_matrix=r["coefficients"], electronic_energy=r["electronic_energy"],
+        total_energy=r["total_energy"], iterations=r["iterations"], n_electrons=r["n_electrons"],
+        n_basis=r["n_basis"], history=r["history"], scf_wall_ms=r["scf_wall_ms"], converged=(st == abi.FOCK_OK))
+
+
+class SelfConsistentField:
+    """Blanket-impl analogue: SelfConsistentField(atoms).try_scf(max_iters, epsilon, charge)."""
+
+    def __init__(self, atoms: Iterable[Atom]):
+        self.atoms = list(atoms)
+
+    def try_scf(self, max_iters: int, epsilon: float, molecule_charge: int = 0, **kw):
+        return try_scf(self.atoms, max_iters, epsilon, molecule_charge, **kw)

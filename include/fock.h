@@ -1,0 +1,230 @@
+/*
+ * fock.h -- C ABI of the MI355X restricted Hartree-Fock engine (libfock.so).
+ *
+ * Plain C17 types only: pointers, sizes, enums. No HIP or torch types appear in
+ * any signature; a HIP stream is passed as an opaque `void *`.
+ *
+ * Each entry point names the fock-rs interface it replaces (paths relative to
+ * the fock-rs repository). A Rust `fock-sys` binding of these declarations is
+ * given in INTEGRATION.md.
+ *
+ * Threading: a context is owned by one host thread at a time and holds no global
+ * mutable state; distinct contexts may run concurrently (unlike the reference's
+ * `static mut BUFFER`, scf/src/utils.rs:7,63).
+ */
+#ifndef FOCK_H
+#define FOCK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef enum fock_status {
+    FOCK_OK = 0,
+    /* Iteration limit reached; outputs are filled. The reference returns
+     * Some(result) in this case (scf/src/lib.rs:224). */
+    FOCK_NOT_CONVERGED = 1,
+    /* max|FDS - SDF| became NaN/inf/subnormal; the reference returns None
+     * (scf/src/lib.rs:248-249). Outputs are not meaningful. */
+    FOCK_DIVERGED = 2,
+    FOCK_INVALID_ARG = 3,
+    /* Boys order > 10 under the compatibility policy; the reference panics
+     * (boys/src/lib.rs:59-62). */
+    FOCK_BOYS_ORDER = 4,
+    FOCK_HIP_ERROR = 5,
+    FOCK_OOM = 6,
+    /* Angular momentum beyond the compiled kernels (s and p shells). */
+    FOCK_UNSUPPORTED = 7
+} fock_status;
+
+/* Integral arithmetic. REF_COMPAT reproduces the reference's own numbers: the
+ * Boys lookup table with linear interpolation (boys/src/lib.rs:58-81) and the
+ * Hermite-Coulomb x-branch exactly as written (basis/src/utils.rs:98-105).
+ * ACCURATE uses a Taylor-table Boys function and the textbook recursion. */
+typedef enum fock_integral_policy {
+    FOCK_POLICY_REF_COMPAT = 0,
+    FOCK_POLICY_ACCURATE = 1
+} fock_integral_policy;
+
+/* Integral screening. REF_BF_PAIR: an integral (xy|zw) is kept iff
+ * sqrt((xy|xy)(zw|zw)) > 1e-6 per basis-function pair, else it is zero
+ * (basis/src/electron_tensor.rs:134-155). SCHWARZ: shell-quartet Schwarz bound
+ * Q_ab Q_cd > threshold. NONE: every quartet. */
+typedef enum fock_screen {
+    FOCK_SCREEN_REF_BF_PAIR = 0,
+    FOCK_SCREEN_SCHWARZ = 1,
+    FOCK_SCREEN_NONE = 2
+} fock_screen;
+
+/* Arithmetic of the quartet integrals. MIXED: quartet integrals and their
+ * contraction in FP32, Fock accumulation, Boys for large arguments, density,
+ * transforms and diagonalisation in FP64. */
+typedef enum fock_precision {
+    FOCK_PRECISION_FP64 = 0,
+    FOCK_PRECISION_MIXED = 1
+} fock_precision;
+
+typedef struct fock_ctx fock_ctx;
+
+/* Flat view of the reference's Vec<ContractedGaussian> (basis/src/contracted_gaussian.rs:9-35),
+ * borrowed for the duration of a call. Function f has centre centers[3f..3f+2] (Bohr),
+ * angular momentum angular[3f..3f+2] and primitives prim_offset[f] .. prim_offset[f+1]-1 with
+ * exponents[] and coefficients[] as GaussianPrimitive::new stores them (already multiplied by
+ * the primitive norm, basis/src/primitives.rs:41-48). Functions must come in complete shells in
+ * the reference's component order (basis_set/src/lib.rs:87-99: p as z, y, x). */
+typedef struct fock_basis_desc {
+    int32_t n_functions;
+    const double *centers;
+    const int32_t *angular;
+    const int32_t *prim_offset;
+    const double *exponents;
+    const double *coefficients;
+} fock_basis_desc;
+
+/* Point charges (basis/src/lib.rs:7-11 PointCharge; basis_set/src/atom.rs:32-43). */
+typedef struct fock_nuclei_desc {
+    int32_t n_atoms;
+    const double *positions; /* [n_atoms][3], Bohr */
+    const double *charges;   /* [n_atoms]; the electron count is sum(charges) - molecule_charge */
+} fock_nuclei_desc;
+
+typedef struct fock_scf_opts {
+    int32_t max_iters;        /* try_scf max_iters: iterations 0..=max_iters */
+    double epsilon;           /* try_scf epsilon on max|FDS - SDF| */
+    int32_t molecule_charge;  /* try_scf molecule_charge */
+    int32_t integral_policy;  /* fock_integral_policy */
+    int32_t screen;           /* fock_screen */
+    double schwarz_threshold; /* for FOCK_SCREEN_SCHWARZ */
+    int32_t precision;        /* fock_precision */
+    /* MIXED only: build G(P_k) = G(P_{k-1}) + G(P_k - P_{k-1}) with density-weighted screening,
+     * then one final full FP64 build at the converged density. */
+    int32_t incremental;
+    int32_t reserved[6];
+} fock_scf_opts;
+
+/* Per-iteration record (optional output). */
+typedef struct fock_scf_iter {
+    double total_energy;
+    double error;           /* max|FDS - SDF| */
+    double fock_build_ms;   /* GPU time of this iteration's Fock build */
+    int64_t quartets;       /* shell quartets evaluated in this build */
+} fock_scf_iter;
+
+/* Outputs of try_scf (scf/src/lib.rs:27-37 HartreeFockResult). Matrices are caller-allocated,
+ * column-major n x n (nalgebra DMatrix layout), rows in the basis-function order of the input;
+ * any pointer may be NULL to skip that output. */
+typedef struct fock_scf_out {
+    double *coefficients;      /* coefficient_matrix: column k = orbital k */
+    double *density;           /* density_matrix */
+    double *orbital_energies;  /* ascending */
+    double electronic_energy;
+    double total_energy;
+    int32_t iterations;        /* 0-based index of the returning iteration */
+    int32_t n_electrons;
+    int32_t n_basis;
+    double last_error;
+    fock_scf_iter *history;    /* optional, capacity history_capacity */
+    int32_t history_capacity;
+    int32_t history_len;
+    double scf_wall_ms;        /* upload of the inputs through the converged energy */
+} fock_scf_out;
+
+/* ----------------------------------------------------------------- context */
+
+/* New context on HIP device `device` with its own stream (default priority). */
+fock_status fock_ctx_create(int device, fock_ctx **out);
+void fock_ctx_destroy(fock_ctx *ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char *fock_ctx_last_error(const fock_ctx *ctx);
+/* Launch on a caller-provided stream (a hipStream_t cast to void*); NULL restores the
+ * context's own stream. */
+fock_status fock_ctx_set_stream(fock_ctx *ctx, void *hip_stream);
+/* Library version string. */
+const char *fock_version(void);
+
+/* ------------------------------------------------------------- the SCF drop-in */
+
+/* RefCompat defaults: max_iters 100, epsilon 1e-6, charge 0, REF_COMPAT, REF_BF_PAIR, FP64. */
+void fock_scf_opts_default(fock_scf_opts *opts);
+
+/* Restricted Hartree-Fock; replaces scf::SelfConsistentField::try_scf
+ * (scf/src/lib.rs:48-78, body :73-258). Returns FOCK_OK (converged), FOCK_NOT_CONVERGED
+ * (iteration limit, outputs filled, the reference's Some), FOCK_DIVERGED (the reference's
+ * None) or an error. */
+fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
+                         const fock_scf_opts *opts, fock_scf_out *out);
+
+/* --------------------------------------------------------- operator surfaces */
+
+/* Boys function F_n(T) for nT arguments; replaces boys::boys (boys/src/lib.rs:58).
+ * policy: fock_integral_policy. T and out are host arrays. n <= 10 for REF_COMPAT
+ * (else FOCK_BOYS_ORDER), n <= 16 for ACCURATE. */
+fock_status fock_boys(fock_ctx *ctx, const double *T, int64_t nT, int32_t n, int32_t policy, double *out);
+
+/* Every ERI (ij|kl) of the basis into out[((i n + j) n + k) n + l] (host, n^4 doubles),
+ * screened as requested; replaces ElectronTensor::from_basis + Index<(usize,usize,usize,usize)>
+ * (basis/src/electron_tensor.rs:81-187). Intended for small n (test hook). */
+fock_status fock_eri_dense(fock_ctx *ctx, const fock_basis_desc *basis, int32_t policy, int32_t screen,
+                           double schwarz_threshold, double *out_n4);
+
+/* Overlap, kinetic and nuclear-attraction matrices (host, row-major n x n, symmetric);
+ * replaces BasisFunction::{overlap_int, kinetic_int, nuclear_attraction_int}
+ * (basis/src/contracted_gaussian.rs:209-262) as assembled by scf/src/lib.rs:118-129.
+ * Any output pointer may be NULL. */
+fock_status fock_one_electron(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
+                              int32_t policy, double *S, double *T, double *V);
+
+/* Prepare direct J/K builds for a basis: shell pairs, Schwarz bounds, screened task lists.
+ * The prepared state lives in the context until the next prepare or destroy. */
+fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *basis, int32_t policy, int32_t screen,
+                            double schwarz_threshold, int32_t precision);
+
+/* J_ij = sum_kl (ij|kl) P_kl and K_ij = sum_kl (ik|jl) P_kl, so that G = J - K/2 is the
+ * reference's two-electron Fock term (scf/src/lib.rs:167-185). Host arrays, row-major n x n;
+ * P must be symmetric. J or K may be NULL. Synchronous. */
+fock_status fock_jk_build(fock_ctx *ctx, const double *P, double *J, double *K);
+
+/* Same on device arrays (HBM), enqueued on the context's stream; no host synchronisation. */
+fock_status fock_jk_build_device(fock_ctx *ctx, const double *d_P, double *d_J, double *d_K);
+
+/* Dense FP64 product on the matrix cores, the primitive of the SCF's dense steps
+ * (nalgebra products at scf/src/lib.rs:141,145,155,157,191,210,212,223):
+ * C = alpha op(A) op(B) + beta C with row-major host arrays, op = transpose when trans* != 0;
+ * op(A) is m x k, op(B) is k x n, C is m x n. */
+fock_status fock_gemm(fock_ctx *ctx, int32_t transA, int32_t transB, int32_t m, int32_t n, int32_t k, double alpha,
+                      const double *A, int32_t lda, const double *B, int32_t ldb, double beta, double *C,
+                      int32_t ldc);
+
+/* Eigen-decomposition of a real symmetric n x n matrix on the GPU with eigenvalues ascending;
+ * replaces scf::utils::sorted_eigs (scf/src/utils.rs:59-99). M, vectors: column-major host
+ * arrays (vectors column k = eigenvector k); values: n. Eigenvector signs are arbitrary. */
+fock_status fock_sorted_eigs(fock_ctx *ctx, int32_t n, const double *M, double *vectors, double *values);
+
+/* DIIS extrapolation; replaces scf::utils::diis (scf/src/utils.rs:102-138): B_ij = <e_i, e_j>
+ * (Frobenius) bordered by -1, solve B c = (0,...,0,-1), out = sum_i c_i F_i. errors and focks
+ * hold m matrices of len doubles each (host). Returns FOCK_INVALID_ARG when B is singular (the
+ * reference's None). */
+fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const double *errors, const double *focks, double *out);
+
+/* Statistics of the prepared J/K build, per quartet class (ss|ss), (ps|ss), (ps|ps), (pp|ss),
+ * (pp|ps), (pp|pp). Counts are of work that survives the screen. */
+typedef struct fock_jk_stats {
+    int32_t n_basis, n_shells, n_shell_pairs;
+    int64_t shell_quartets[6];
+    int64_t primitive_quartets[6];
+    int64_t integrals[6];        /* contracted Cartesian integrals (shell quartets x components) */
+    int64_t tasks[6];            /* wavefront tasks */
+    double algorithmic_flop[6];  /* canonical McMurchie-Davidson count: f_prim(class) per primitive
+                                  * quartet + 12 per contracted integral (J: 2 FMA, K: 4 FMA) */
+    int64_t near_threshold;      /* REF_BF_PAIR: |estimate - 1e-6| < 1e-12 pairs met */
+} fock_jk_stats;
+fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FOCK_H */

@@ -1,0 +1,120 @@
+"""Generate the golden fixtures under tests/golden/ from the CPU oracle.
+
+    python tests/golden/make_golden.py small     # H2, H2O (W7), H2O/6-31G: seconds
+    python tests/golden/make_golden.py o108      # n-octane/6-31G: ~1 minute
+    python tests/golden/make_golden.py e264      # n-eicosane/6-31G accurate energy: ~15 minutes
+
+Fixtures are plain .npz arrays (no pickles). The oracle is the checker: refcpu.c
+(the reference's arithmetic) and oracle64.c (accurate, unscreened), with the SCF
+restated in oracle/refscf.py.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "fock-rs_amd"))
+
+from fock_amd import molecule as m  # noqa: E402
+from oracle import refscf  # noqa: E402
+
+
+def rot_y(deg):
+    t = np.radians(deg)
+    return np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+
+
+def random_density(n, nocc, seed):
+    """A valid idempotent-like density: 2 C_occ C_occ^T from random orthonormal columns."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n))
+    Q, _ = np.linalg.qr(A)
+    C = Q[:, :nocc] * 0.3
+    return np.ascontiguousarray(2.0 * C @ C.T)
+
+
+def scf_both(fb, fn, eps, tag, out, max_iters=200):
+    for acc in (False, True):
+        t = time.time()
+        S, T, V = refscf.one_electron(fb, fn, acc)
+        packed, computed = refscf.eri_packed(fb, acc, screen=True)
+        r = refscf.try_scf(fb, fn, max_iters, eps, 0, accurate=acc, integrals=(S, T, V, packed))
+        key = tag + ("_acc" if acc else "_ref")
+        out[key + "_energy"] = np.array([r.total_energy])
+        out[key + "_eelec"] = np.array([r.electronic_energy])
+        out[key + "_eps"] = r.orbital_energies
+        out[key + "_P"] = r.density_matrix
+        out[key + "_iters"] = np.array([r.iterations])
+        out[key + "_computed"] = np.array([computed])
+        print("%s: E=%.12f iters=%d (%.1fs)" % (key, r.total_energy, r.iterations, time.time() - t), flush=True)
+        yield acc, S, T, V, packed
+
+
+def small():
+    out = {}
+    # W7: H2O/STO-3G, the reference-CPU oracle case
+    fb, fn = m.flatten(m.h2o_sto3g())
+    for acc, S, T, V, packed in scf_both(fb, fn, 1e-10, "w7", out):
+        k = "w7_acc" if acc else "w7_ref"
+        out[k + "_S"], out[k + "_T"], out[k + "_V"] = S, T, V
+        out[k + "_eri"] = refscf.dense_from_packed(fb.n, packed)
+        if not acc:
+            unscreened, _ = refscf.eri_packed(fb, False, screen=False)
+            out["w7_ref_eri_unscreened"] = refscf.dense_from_packed(fb.n, unscreened)
+    # the same molecule rotated 90 degrees about y: exposes the x-branch defect
+    fb, fn = m.flatten(m.build(m.rotate_geometry(m.water_geometry(), rot_y(90.0)), "STO-3G"))
+    for acc, S, T, V, packed in scf_both(fb, fn, 1e-10, "w7rot", out):
+        k = "w7rot_acc" if acc else "w7rot_ref"
+        out[k + "_eri"] = refscf.dense_from_packed(fb.n, packed)
+    # H2O/6-31G (n = 13) and H2/STO-3G at 1.4 bohr (Szabo-Ostlund)
+    fb, fn = m.flatten(m.build(m.water_geometry(), "6-31G"))
+    for acc, S, T, V, packed in scf_both(fb, fn, 1e-10, "w13", out):
+        k = "w13_acc" if acc else "w13_ref"
+        out[k + "_eri"] = refscf.dense_from_packed(fb.n, packed)
+    b = m.BasisSet.load("STO-3G")
+    fb, fn = m.flatten([b.get((0, 0, 0), 1), b.get((0, 0, 1.4), 1)])
+    for acc, S, T, V, packed in scf_both(fb, fn, 1e-10, "h2", out):
+        pass
+    np.savez_compressed(os.path.join(HERE, "small.npz"), **out)
+
+
+def o108():
+    out = {}
+    fb, fn = m.flatten(m.octane_631g())
+    n = fb.n
+    P = random_density(n, 33, seed=108)
+    out["P_random"] = P
+    for acc, S, T, V, packed in scf_both(fb, fn, 1e-9, "o108", out):
+        J = np.zeros((n, n)); K = np.zeros((n, n))
+        from oracle import dptr, lib
+        lib().o64_jk_packed(n, dptr(packed), dptr(P), dptr(J), dptr(K))
+        k = "o108_acc" if acc else "o108_ref"
+        out[k + "_J"], out[k + "_K"] = J, K
+        out[k + "_S"], out[k + "_T"], out[k + "_V"] = S, T, V
+    np.savez_compressed(os.path.join(HERE, "o108.npz"), **out)
+
+
+def e264():
+    out = {}
+    fb, fn = m.flatten(m.eicosane_631g())
+    t = time.time()
+    S, T, V = refscf.one_electron(fb, fn, True)
+    packed, _ = refscf.eri_packed(fb, True)
+    print("e264 accurate integrals %.1fs" % (time.time() - t), flush=True)
+    r = refscf.try_scf(fb, fn, 200, 1e-8, 0, accurate=True, integrals=(S, T, V, packed))
+    out["e264_acc_energy"] = np.array([r.total_energy])
+    out["e264_acc_eelec"] = np.array([r.electronic_energy])
+    out["e264_acc_eps"] = r.orbital_energies
+    out["e264_acc_iters"] = np.array([r.iterations])
+    out["e264_acc_errors"] = np.array(r.errors)
+    print("e264 accurate E=%.12f iters=%d (%.1fs)" % (r.total_energy, r.iterations, time.time() - t), flush=True)
+    np.savez_compressed(os.path.join(HERE, "e264.npz"), **out)
+
+
+if __name__ == "__main__":
+    for stage in sys.argv[1:] or ["small"]:
+        {"small": small, "o108": o108, "e264": e264}[stage]()

@@ -1,0 +1,200 @@
+"""GPU parity: libfock.so (through the C ABI) against the CPU oracle and golden fixtures.
+
+Tolerances (north-star): ERIs within 1e-10, total energy within 1e-8 Eh for s/p basis
+sets; Boys within rounding of the restated algorithm.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+from fock_amd import abi, molecule as m
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(REPO, "tests", "golden")
+
+
+@pytest.fixture(scope="module")
+def small():
+    return np.load(os.path.join(GOLD, "small.npz"))
+
+
+@pytest.fixture(scope="module")
+def o108():
+    return np.load(os.path.join(GOLD, "o108.npz"))
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------------------ Boys
+
+def test_boys_ref_policy_matches_restatement(ctx, oracle_lib):
+    say("boys ref policy")
+    T = np.concatenate([[0.0, 1e-17, 1e-16, 1e-10, 0.15, 0.3, 29.85, 29.9, 29.999, 30.0, 30.5, 45.0, 100.0, 1e4],
+                        np.linspace(0.0, 35.0, 3501)])
+    for n in range(11):
+        got = ctx.boys(T, n, abi.POLICY_REF_COMPAT)
+        want = np.array([oracle_lib.ref_boys(float(t), n) for t in T])
+        rel = np.abs(got - want) / np.abs(want)
+        assert rel.max() < 1e-14, (n, rel.max(), T[rel.argmax()])
+
+
+def test_boys_ref_policy_order_limit(ctx):
+    with pytest.raises(abi.FockError) as e:
+        ctx.boys(np.array([1.0]), 11, abi.POLICY_REF_COMPAT)
+    assert e.value.status == abi.FOCK_BOYS_ORDER
+
+
+def test_boys_accurate_matches_oracle64(ctx, oracle_lib):
+    say("boys accurate policy")
+    T = np.concatenate([np.linspace(0.0, 60.0, 6001), [1e-12, 44.999, 45.0, 45.0001, 200.0, 1e5]])
+    for n in range(12):
+        got = ctx.boys(T, n, abi.POLICY_ACCURATE)
+        want = np.array([oracle_lib.o64_boys(float(t), n) for t in T])
+        rel = np.abs(got - want) / np.abs(want)
+        assert rel.max() < 5e-14, (n, rel.max(), T[rel.argmax()])
+
+
+def test_boys_accurate_rejects_negative_argument(ctx):
+    with pytest.raises(abi.FockError):
+        ctx.boys(np.array([-1.0]), 0, abi.POLICY_ACCURATE)
+
+
+# ------------------------------------------------------------------ one-electron
+
+@pytest.mark.parametrize("policy,key", [(abi.POLICY_REF_COMPAT, "w7_ref"), (abi.POLICY_ACCURATE, "w7_acc")])
+def test_one_electron_w7(ctx, small, policy, key):
+    say("one-electron " + key)
+    fb, fn = m.flatten(m.h2o_sto3g())
+    S, T, V = ctx.one_electron(fb, fn, policy)
+    for name, got in (("S", S), ("T", T), ("V", V)):
+        want = small[key + "_" + name]
+        assert np.abs(got - want).max() < 1e-12, (name, np.abs(got - want).max())
+
+
+@pytest.mark.parametrize("policy,key", [(abi.POLICY_REF_COMPAT, "o108_ref"), (abi.POLICY_ACCURATE, "o108_acc")])
+def test_one_electron_o108(ctx, o108, policy, key):
+    say("one-electron " + key)
+    fb, fn = m.flatten(m.octane_631g())
+    S, T, V = ctx.one_electron(fb, fn, policy)
+    for name, got in (("S", S), ("T", T), ("V", V)):
+        want = o108[key + "_" + name]
+        assert np.abs(got - want).max() < 1e-11, (name, np.abs(got - want).max())
+
+
+# ------------------------------------------------------------------ ERIs
+
+def _rot_y(deg):
+    t = np.radians(deg)
+    return np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+
+
+@pytest.mark.parametrize("case", ["w7", "w7rot", "w13"])
+def test_eri_dense_ref_compat(ctx, small, case):
+    say("dense eri ref " + case)
+    if case == "w7":
+        atoms = m.h2o_sto3g()
+    elif case == "w7rot":
+        atoms = m.build(m.rotate_geometry(m.water_geometry(), _rot_y(90.0)), "STO-3G")
+    else:
+        atoms = m.build(m.water_geometry(), "6-31G")
+    fb, _ = m.flatten(atoms)
+    got = ctx.eri_dense(fb, abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR)
+    want = small[case + "_ref_eri"]
+    assert np.abs(got - want).max() < 1e-10, np.abs(got - want).max()
+
+
+def test_eri_dense_ref_compat_unscreened(ctx, small):
+    fb, _ = m.flatten(m.h2o_sto3g())
+    got = ctx.eri_dense(fb, abi.POLICY_REF_COMPAT, abi.SCREEN_NONE)
+    want = small["w7_ref_eri_unscreened"]
+    assert np.abs(got - want).max() < 1e-10
+
+
+@pytest.mark.parametrize("case", ["w7", "w7rot", "w13"])
+def test_eri_dense_accurate(ctx, small, case):
+    say("dense eri accurate " + case)
+    if case == "w7":
+        atoms = m.h2o_sto3g()
+    elif case == "w7rot":
+        atoms = m.build(m.rotate_geometry(m.water_geometry(), _rot_y(90.0)), "STO-3G")
+    else:
+        atoms = m.build(m.water_geometry(), "6-31G")
+    fb, _ = m.flatten(atoms)
+    got = ctx.eri_dense(fb, abi.POLICY_ACCURATE, abi.SCREEN_NONE)
+    want = small[case + "_acc_eri"]
+    assert np.abs(got - want).max() < 1e-10, np.abs(got - want).max()
+
+
+# ------------------------------------------------------------------ J/K
+
+@pytest.mark.parametrize("policy,screen,key,tol", [
+    (abi.POLICY_ACCURATE, abi.SCREEN_NONE, "o108_acc", 1e-10),
+    (abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, "o108_acc", 1e-9),
+    (abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR, "o108_ref", 1e-10),
+])
+def test_jk_o108(ctx, o108, policy, screen, key, tol):
+    say("jk o108 %s screen=%d" % (key, screen))
+    fb, _ = m.flatten(m.octane_631g())
+    ctx.jk_prepare(fb, policy, screen, 1e-12)
+    J, K = ctx.jk_build(o108["P_random"])
+    assert np.abs(J - o108[key + "_J"]).max() < tol, np.abs(J - o108[key + "_J"]).max()
+    assert np.abs(K - o108[key + "_K"]).max() < tol, np.abs(K - o108[key + "_K"]).max()
+    st = ctx.jk_stats()
+    assert st["n_basis"] == 108 and st["n_shells"] == 76
+    if screen == abi.SCREEN_NONE:
+        assert sum(st["shell_quartets"]) == 4282201  # unique shell quartets (s and p of SP counted apart)
+
+
+def test_screen_lists_are_bit_exact_between_runs(ctx):
+    fb, _ = m.flatten(m.octane_631g())
+    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12)
+    a = ctx.jk_stats()
+    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12)
+    b = ctx.jk_stats()
+    assert a == b
+
+
+# ------------------------------------------------------------------ SCF
+
+@pytest.mark.parametrize("key,policy,screen,tol", [
+    ("w7_ref", abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR, 1e-8),
+    ("w7_acc", abi.POLICY_ACCURATE, abi.SCREEN_NONE, 1e-8),
+    ("h2_acc", abi.POLICY_ACCURATE, abi.SCREEN_NONE, 1e-8),
+    ("w13_ref", abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR, 1e-8),
+])
+def test_scf_small(ctx, small, key, policy, screen, tol):
+    say("scf " + key)
+    from fock_amd.scf import try_scf
+    if key.startswith("w7"):
+        atoms = m.h2o_sto3g()
+    elif key.startswith("w13"):
+        atoms = m.build(m.water_geometry(), "6-31G")
+    else:
+        b = m.BasisSet.load("STO-3G")
+        atoms = [b.get((0, 0, 0), 1), b.get((0, 0, 1.4), 1)]
+    r = try_scf(atoms, 200, 1e-10, 0, policy=policy, screen=screen, ctx=ctx)
+    assert r is not None and r.converged
+    assert abs(r.total_energy - small[key + "_energy"][0]) < tol, (r.total_energy, small[key + "_energy"][0])
+    assert np.abs(r.orbital_energies - small[key + "_eps"]).max() < 1e-7
+    assert np.abs(r.density_matrix - small[key + "_P"]).max() < 1e-6
+
+
+def test_scf_o108_ref_compat(ctx, o108):
+    say("scf o108 ref")
+    from fock_amd.scf import try_scf
+    r = try_scf(m.octane_631g(), 200, 1e-9, 0, ctx=ctx)
+    assert r is not None and r.converged
+    assert abs(r.total_energy - o108["o108_ref_energy"][0]) < 1e-8, (r.total_energy, o108["o108_ref_energy"][0])
+
+
+def test_scf_o108_accurate_schwarz(ctx, o108):
+    say("scf o108 accurate")
+    from fock_amd.scf import try_scf
+    r = try_scf(m.octane_631g(), 200, 1e-9, 0, policy=abi.POLICY_ACCURATE, screen=abi.SCREEN_SCHWARZ,
+                schwarz_threshold=1e-12, ctx=ctx)
+    assert r is not None and r.converged
+    assert abs(r.total_energy - o108["o108_acc_energy"][0]) < 1e-8, (r.total_energy, o108["o108_acc_energy"][0])

@@ -1,0 +1,300 @@
+"""Pins of the CPU oracle (oracle/refcpu.c, oracle/oracle64.c, oracle/refscf.py).
+
+Each check is chosen so a plausible restatement mistake (dropped term, wrong sign or index,
+transposed operand) fails it: reference golden values, closed forms, mpmath at 40 digits,
+derivative identities, and agreement between the two independent implementations.
+"""
+import ctypes
+import math
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from conftest import REPO
+from fock_amd import molecule as m
+
+mp.mp.dps = 40
+GOLD = os.path.join(REPO, "tests", "golden")
+
+
+def boys_mp(T, n):
+    T = mp.mpf(T)
+    if T == 0:
+        return mp.mpf(1) / (2 * n + 1)
+    return mp.hyp1f1(n + mp.mpf(1) / 2, n + mp.mpf(3) / 2, -T) / (2 * n + 1)
+
+
+# ------------------------------------------------------------------ reference goldens
+
+def test_hermite_expansion_doctest_bit_exact(oracle_lib):
+    # basis/src/utils.rs:24
+    assert oracle_lib.ref_hermite_expansion(1, 2, 3, 2.5, 3.0, 5.0) == 1.9872458203327215e-9
+
+
+def test_lut_is_f10_on_the_0p15_grid():
+    vals = [float(x) for x in open(os.path.join(REPO, "fock-rs_amd", "csrc", "ref_boys_lut.inc")).read()
+            .split("*/")[1].replace("\n", "").split(",") if x.strip()]
+    assert len(vals) == 200
+    for k in range(0, 200, 7):
+        want = boys_mp(0.15 * k, 10)
+        assert abs(vals[k] - want) / want < 4e-13
+
+
+# ------------------------------------------------------------------ Boys
+
+def test_ref_boys_regimes(oracle_lib):
+    f = oracle_lib.ref_boys
+    for n in range(11):
+        assert f(0.0, n) == 1.0 / (2 * n + 1)          # t < 1e-16 branch
+        for T in (30.0, 31.7, 100.0):                   # asymptotic branch, boys/src/lib.rs:66-69
+            df = 1
+            for k in range(2 * n - 1, 0, -2):
+                df *= k
+            want = df * (1.0 / 2 ** (n + 1)) * math.sqrt(math.pi * oracle_lib.ref_powi(T, -2 * n - 1))
+            assert f(T, n) == want
+    # the quirks the reference documents: ~1e-3 relative error and a jump at T = 30
+    errs = [abs(f(T, 0) - float(boys_mp(T, 0))) / float(boys_mp(T, 0)) for T in np.linspace(0.01, 29.8, 60)]
+    assert 1e-6 < max(errs) < 2e-2
+    jump = abs(f(29.999, 0) - f(30.0, 0)) / f(30.0, 0)
+    assert 0.02 < jump < 0.06
+    # downward recursion identity F_{n-1} = (2T F_n + e^-T)/(2n-1) holds exactly for the LUT branch
+    T = 7.3
+    for n in range(1, 11):
+        lhs = f(T, n - 1)
+        rhs = (2.0 * T * f(T, n) + math.exp(-T)) / (2 * n - 1)
+        assert lhs == rhs
+
+
+def test_oracle64_boys_vs_mpmath(oracle_lib):
+    for n in (0, 1, 2, 4, 7, 11):
+        for T in (0.0, 1e-9, 0.37, 3.0, 14.9, 29.99, 44.9, 45.1, 80.0, 400.0):
+            want = boys_mp(T, n)
+            got = oracle_lib.o64_boys(T, n)
+            assert abs(got - want) / want < 2e-15, (n, T)
+
+
+def test_ref_boys_order_limit_flags(oracle_lib):
+    assert math.isnan(oracle_lib.ref_boys(1.0, 11))
+    assert oracle_lib.ref_boys_order_error() == 1
+
+
+# ------------------------------------------------------------------ Hermite Coulomb R
+
+def _R(lib, t, u, v, n, p, d):
+    arr = (ctypes.c_double * 3)(*d)
+    return lib.ref_coulomb_auxiliary(t, u, v, n, p, arr, d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+
+
+def test_x_branch_defect_closed_form(oracle_lib):
+    """basis/src/utils.rs:98-105 drops (t-1): R_bug(3,0,0) - R_ok(3,0,0) = -R^{(1)}_{100}
+    = -X (-2p)^2 F_2(T) for n = 0 (and likewise shifted for n > 0)."""
+    L = oracle_lib
+    p, d = 0.83, (0.7, -0.4, 1.1)
+    T = p * (d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+    try:
+        for n in (0, 1):
+            L.ref_set_policy(1, 0)
+            bug = _R(L, 3, 0, 0, n, p, d)
+            L.ref_set_policy(1, 1)
+            ok = _R(L, 3, 0, 0, n, p, d)
+            want = -d[0] * (-2 * p) ** (n + 2) * L.o64_boys(T, n + 2)
+            assert abs((bug - ok) - want) < 1e-14
+            # t <= 2 and every y/z branch are unaffected
+            for tuv in [(2, 0, 0), (1, 1, 0), (0, 0, 3), (0, 3, 0), (2, 1, 1), (0, 2, 2)]:
+                L.ref_set_policy(1, 0)
+                a = _R(L, *tuv, n, p, d)
+                L.ref_set_policy(1, 1)
+                b = _R(L, *tuv, n, p, d)
+                assert a == b
+        # axis permutation symmetry R_400(X,Y,Z) = R_004(Z,Y,X): holds for the correct recursion only
+        L.ref_set_policy(1, 1)
+        assert abs(_R(L, 4, 0, 0, 0, p, d) - _R(L, 0, 0, 4, 0, p, (d[2], d[1], d[0]))) < 1e-14
+        L.ref_set_policy(1, 0)
+        assert abs(_R(L, 4, 0, 0, 0, p, d) - _R(L, 0, 0, 4, 0, p, (d[2], d[1], d[0]))) > 1e-3
+    finally:
+        L.ref_set_policy(0, 0)
+
+
+# ------------------------------------------------------------------ integrals vs closed forms
+
+def _flat(prims):
+    """prims: list of (centre, angular, exponent) with coefficient 1 (unnormalised)."""
+    fb = m.FlatBasis(
+        np.ascontiguousarray(np.array([c for c, _, _ in prims], dtype=np.float64)),
+        np.ascontiguousarray(np.array([a for _, a, _ in prims], dtype=np.int32)),
+        np.ascontiguousarray(np.arange(len(prims) + 1, dtype=np.int32)),
+        np.ascontiguousarray(np.array([e for _, _, e in prims], dtype=np.float64)),
+        np.ascontiguousarray(np.ones(len(prims))),
+    )
+    return fb
+
+
+def ssss_mp(A, a, B, b, C, c, D, d):
+    A, B, C, D = [mp.matrix(x) for x in (A, B, C, D)]
+    a, b, c, d = map(mp.mpf, (a, b, c, d))
+    p, q = a + b, c + d
+    P = (a * A + b * B) / p
+    Q = (c * C + d * D) / q
+    nrm = lambda v: sum(x ** 2 for x in v)
+    Kab = mp.e ** (-a * b / p * nrm(A - B))
+    Kcd = mp.e ** (-c * d / q * nrm(C - D))
+    alpha = p * q / (p + q)
+    return 2 * mp.pi ** mp.mpf(2.5) / (p * q * mp.sqrt(p + q)) * Kab * Kcd * boys_mp(alpha * nrm(P - Q), 0)
+
+
+GEOM = [((0.1, -0.2, 0.3), 1.3), ((1.0, 0.4, -0.5), 0.7), ((-0.6, 0.9, 0.2), 2.1), ((0.3, 1.4, 1.0), 0.45)]
+
+
+def _oracle_quartet(oracle_lib, angs):
+    """One primitive ERI from both oracles: refcpu (per function, accurate switches on) and
+    oracle64 (per complete shell, the requested component picked out)."""
+    import oracle
+    prims, first, comp = [], [], []
+    for i, ang in enumerate(angs):
+        L = sum(ang)
+        comps = m.cartesian_components(L)
+        first.append(len(prims))
+        comp.append(comps.index(tuple(ang)))
+        for c in comps:
+            prims.append((GEOM[i][0], c, GEOM[i][1]))
+    fb = _flat(prims)
+    bv = oracle.BasisView(fb)
+    oracle_lib.ref_set_policy(1, 1)
+    try:
+        ref = oracle_lib.ref_eri(bv.ref, first[0] + comp[0], first[1] + comp[1], first[2] + comp[2],
+                                 first[3] + comp[3])
+    finally:
+        oracle_lib.ref_set_policy(0, 0)
+    nf = [len(m.cartesian_components(sum(a))) for a in angs]
+    out = np.zeros(int(np.prod(nf)))
+    assert oracle_lib.o64_shell_quartet(bv.ref, first[0], first[1], first[2], first[3], oracle.dptr(out)) == 0
+    idx = ((comp[0] * nf[1] + comp[1]) * nf[2] + comp[2]) * nf[3] + comp[3]
+    return ref, out[idx]
+
+
+def test_ssss_closed_form(oracle_lib):
+    want = ssss_mp(GEOM[0][0], GEOM[0][1], GEOM[1][0], GEOM[1][1], GEOM[2][0], GEOM[2][1], GEOM[3][0], GEOM[3][1])
+    ref, o64 = _oracle_quartet(oracle_lib, [(0, 0, 0)] * 4)
+    assert abs(ref - want) / want < 1e-13
+    assert abs(o64 - want) / want < 1e-13
+
+
+@pytest.mark.parametrize("angs", [
+    [(1, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)],
+    [(0, 1, 0), (0, 0, 1), (0, 0, 0), (0, 0, 0)],
+    [(1, 0, 0), (0, 0, 0), (0, 1, 0), (0, 0, 0)],
+    [(0, 0, 1), (0, 0, 0), (0, 0, 1), (1, 0, 0)],
+    [(1, 0, 0), (1, 0, 0), (1, 0, 0), (1, 0, 0)],
+])
+def test_p_integrals_are_centre_derivatives(oracle_lib, angs):
+    """A reference p function (A_x - x) e^{-a|r-A|^2} equals -(1/2a) d/dA_x of the s primitive,
+    so every p-containing primitive ERI is a mixed centre derivative of the (ss|ss) closed form."""
+    mp.mp.dps = 30
+    centres = [list(map(mp.mpf, g[0])) for g in GEOM]
+    exps = [mp.mpf(g[1]) for g in GEOM]
+    vars_, orders, scale = [], [], mp.mpf(1)
+    for i, ang in enumerate(angs):
+        for ax in range(3):
+            if ang[ax]:
+                vars_.append((i, ax))
+                orders.append(ang[ax])
+                scale *= -1 / (2 * exps[i])
+
+    def f(*xs):
+        C = [list(c) for c in centres]
+        for (i, ax), x in zip(vars_, xs):
+            C[i][ax] = x
+        return ssss_mp(C[0], exps[0], C[1], exps[1], C[2], exps[2], C[3], exps[3])
+
+    x0 = [centres[i][ax] for i, ax in vars_]
+    want = float(scale * mp.diff(f, tuple(x0), tuple(orders)))
+    ref, o64 = _oracle_quartet(oracle_lib, angs)
+    mp.mp.dps = 40
+    assert abs(o64 - want) < 1e-12 * max(1.0, abs(want)), (o64, want)
+    assert abs(ref - want) < 1e-12 * max(1.0, abs(want)), (ref, want)
+
+
+def test_szabo_ostlund_h2(oracle_lib):
+    """Szabo & Ostlund, Modern Quantum Chemistry, H2/STO-3G at R = 1.4 a0 (4 decimals)."""
+    from oracle import refscf
+    b = m.BasisSet.load("STO-3G")
+    fb, fn = m.flatten([b.get((0, 0, 0), 1), b.get((0, 0, 1.4), 1)])
+    S, T, V = refscf.one_electron(fb, fn, True)
+    packed, _ = refscf.eri_packed(fb, True)
+    D = refscf.dense_from_packed(2, packed)
+    assert round(S[0, 1], 4) == 0.6593
+    assert round(D[0, 0, 0, 0], 4) == 0.7746
+    assert round(D[0, 0, 1, 1], 4) == 0.5697
+    assert round(D[0, 1, 0, 1], 4) == 0.2970
+    assert round(D[0, 0, 0, 1], 4) == 0.4441
+    r = refscf.try_scf(fb, fn, 100, 1e-10, 0, accurate=True)
+    assert round(r.total_energy, 4) == -1.1167
+
+
+def test_refcpu_with_fixed_policies_equals_oracle64(oracle_lib):
+    """The restatement with the accurate Boys and the (t-1) factor restored is an independent
+    second implementation of the accurate oracle: whole W13 tensor and one-electron matrices."""
+    import oracle
+    from oracle import refscf
+    fb, fn = m.flatten(m.build(m.water_geometry(), "6-31G"))
+    oracle_lib.ref_set_policy(1, 1)
+    try:
+        packed_ref = np.zeros(oracle_lib.ref_packed_size(fb.n))
+        oracle_lib.ref_electron_tensor(oracle.BasisView(fb).ref, 0, oracle.dptr(packed_ref))
+        S = np.zeros((fb.n, fb.n)); T = np.zeros_like(S); V = np.zeros_like(S)
+        pos, ch = np.ascontiguousarray(fn.positions), np.ascontiguousarray(fn.charges)
+        oracle_lib.ref_one_electron(oracle.BasisView(fb).ref, len(ch), oracle.dptr(pos), oracle.dptr(ch),
+                                    oracle.dptr(S), oracle.dptr(T), oracle.dptr(V))
+    finally:
+        oracle_lib.ref_set_policy(0, 0)
+    packed_o64, _ = refscf.eri_packed(fb, True)
+    assert np.abs(packed_ref - packed_o64).max() < 1e-13
+    S2, T2, V2 = refscf.one_electron(fb, fn, True)
+    assert np.abs(S - S2).max() < 1e-14 and np.abs(T - T2).max() < 1e-13 and np.abs(V - V2).max() < 1e-12
+
+
+def test_eight_fold_symmetry_and_schwarz(oracle_lib):
+    from oracle import refscf
+    fb, fn = m.flatten(m.h2o_sto3g())
+    packed, _ = refscf.eri_packed(fb, True)
+    D = refscf.dense_from_packed(fb.n, packed)
+    n = fb.n
+    for (i, j, k, l) in [(0, 1, 2, 3), (4, 2, 6, 1), (5, 5, 3, 0)]:
+        v = D[i, j, k, l]
+        for perm in [(j, i, k, l), (i, j, l, k), (k, l, i, j), (l, k, j, i)]:
+            assert D[perm] == v
+    diag = np.array([[D[i, j, i, j] for j in range(n)] for i in range(n)])
+    assert np.all(np.abs(D) <= np.sqrt(diag[:, :, None, None] * diag[None, None, :, :]) + 1e-14)
+
+
+# ------------------------------------------------------------------ SCF restatement
+
+def test_rotation_invariance_accurate_and_defect_ref():
+    g = np.load(os.path.join(GOLD, "small.npz"))
+    assert abs(g["w7_acc_energy"][0] - g["w7rot_acc_energy"][0]) < 1e-8
+    # the reference's x-branch defect makes its energy orientation dependent
+    assert abs(g["w7_ref_energy"][0] - g["w7rot_ref_energy"][0]) > 1e-3
+
+
+def test_scf_invariants_on_fixture():
+    g = np.load(os.path.join(GOLD, "small.npz"))
+    from oracle import refscf
+    fb, fn = m.flatten(m.h2o_sto3g())
+    S, _, _ = refscf.one_electron(fb, fn, True)
+    P = g["w7_acc_P"]
+    assert abs(np.trace(P @ S) - 10.0) < 1e-9        # tr(PS) = N_e
+    assert np.abs(P @ S @ P - 2.0 * P).max() < 1e-8  # PSP = 2P (idempotency)
+    assert np.all(np.diff(g["w7_acc_eps"]) >= 0)     # ascending orbital energies
+
+
+def test_small_fixtures_regenerate_from_oracle():
+    """The committed small fixtures are what the oracle computes now."""
+    from oracle import refscf
+    g = np.load(os.path.join(GOLD, "small.npz"))
+    fb, fn = m.flatten(m.h2o_sto3g())
+    r = refscf.try_scf(fb, fn, 200, 1e-10, 0, accurate=False)
+    assert r.total_energy == pytest.approx(g["w7_ref_energy"][0], abs=1e-10)
+    packed, _ = refscf.eri_packed(fb, False)
+    assert np.abs(refscf.dense_from_packed(fb.n, packed) - g["w7_ref_eri"]).max() == 0.0
