@@ -287,4 +287,18 @@ hipError_t add_f64(int64_t len, const double *a, const double *b, double *out, h
     return hipGetLastError();
 }
 
+__global__ void accumulate_g_kernel(int n, const double *__restrict__ Jb, const double *__restrict__ Kb,
+                                    double *__restrict__ G) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    const int64_t t = (int64_t)j * n + i;
+    G[e] += (Jb[e] + Jb[t]) - 0.5 * (Kb[e] + Kb[t]);
+}
+hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(accumulate_g_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, Jb, Kb, Gacc);
+    return hipGetLastError();
+}
+
 }  // namespace fock

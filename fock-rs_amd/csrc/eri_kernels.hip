@@ -28,10 +28,12 @@ __global__ __launch_bounds__(256) void eri_task_kernel(JKArgs args) {
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= args.ntasks) return;  // wave-uniform
+    if (args.d_ntasks && wave >= *args.d_ntasks) return;
     const Task task = args.tasks[wave];
     const bool active = lane < task.ket_count;
     const PairInfo bra = args.pairs[task.bra];
-    const int ket_idx = task.ket_begin + (active ? lane : 0);
+    const int slot = task.ket_begin + (active ? lane : 0);
+    const int ket_idx = args.ket_list ? args.ket_list[slot] : slot;
     const PairInfo ket = args.pairs[ket_idx];
     const PrimPair<Real> *__restrict__ pp = reinterpret_cast<const PrimPair<Real> *>(args.prims);
 

@@ -83,6 +83,15 @@ struct Prepared {
     DevBuf d_Jb, d_Kb, d_P, d_J, d_K;
     // shell metadata on device (density-weighted screening)
     DevBuf d_shell_f0, d_shell_nf;
+    // incremental builds: per-task group bounds (class-local), pair shells and bounds, buffers
+    DevBuf d_gfirst, d_glast, d_pair_shell, d_pairQ, d_Pm, d_flag32, d_flag64;
+    struct Compact {
+        DevBuf pos, kets, head, tid, kbra, kend, kcost, tasks, count, tmp;
+        size_t tmp_bytes = 0;
+    } c32, c64;
+    int64_t max_slots = 0;
+    bool compact_ready = false;
+    int64_t last_survivors = 0;
 };
 
 }  // namespace
@@ -96,6 +105,8 @@ struct fock_ctx {
     DevBuf lut, tab64, tab32;
     BoysTables bt{};
     Prepared jk;
+    bool timing = false;
+    hipEvent_t tev[NUM_CLASSES + 1] = {};
     fock_status fail(fock_status st, const std::string &msg) {
         err = msg;
         return st;
@@ -293,6 +304,8 @@ extern "C" void fock_ctx_destroy(fock_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t &e : ctx->tev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -495,6 +508,7 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         return di * dj > pr.dthr;
     };
     std::vector<Task> tasks;
+    std::vector<int> gfirst_all, glast_all;
     fock_jk_stats st{};
     st.n_basis = n;
     st.n_shells = ns;
@@ -503,6 +517,8 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         pr.task_off[cls] = (int)tasks.size();
         const int X = CLASS_BRA[cls], Y = CLASS_KET[cls];
         std::vector<Task> ct;
+        std::vector<int> cgid;
+        int ngroup = 0;
         for (int i = 0; i < npair; ++i) {
             if (pr.pairs[i].cls != X) continue;
             const double di = pr.pairs[i].dmax;
@@ -535,13 +551,31 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
                 for (int s = lo; s < hi; s += 64) {
                     const int cnt = std::min(64, hi - s);
                     ct.push_back(Task{i, s, cnt, pr.pairs[i].K * bk.K});
+                    cgid.push_back(ngroup);
                     st.shell_quartets[cls] += cnt;
                     st.primitive_quartets[cls] += (int64_t)cnt * pr.pairs[i].K * bk.K;
                 }
+                if (hi > lo) ++ngroup;
             }
         }
-        // most expensive tasks first (load balance); stable for determinism
-        std::stable_sort(ct.begin(), ct.end(), [](const Task &a, const Task &b) { return a.flags > b.flags; });
+        // most expensive tasks first (load balance); stable for determinism. The tasks of one
+        // (bra, bucket) group share their cost and stay adjacent.
+        std::vector<int> perm(ct.size());
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return ct[a].flags > ct[b].flags; });
+        std::vector<Task> sorted(ct.size());
+        std::vector<int> sg(ct.size());
+        for (size_t q = 0; q < perm.size(); ++q) {
+            sorted[q] = ct[perm[q]];
+            sg[q] = cgid[perm[q]];
+        }
+        ct.swap(sorted);
+        std::vector<int> gf(ct.size()), gl(ct.size());
+        for (size_t q = 0; q < ct.size(); ++q) gf[q] = (q > 0 && sg[q] == sg[q - 1]) ? gf[q - 1] : (int)q;
+        for (size_t q = ct.size(); q-- > 0;) gl[q] = (q + 1 < ct.size() && sg[q] == sg[q + 1]) ? gl[q + 1] : (int)q;
+        gfirst_all.insert(gfirst_all.end(), gf.begin(), gf.end());
+        glast_all.insert(glast_all.end(), gl.begin(), gl.end());
+        pr.max_slots = std::max<int64_t>(pr.max_slots, (int64_t)ct.size() * 64);
         tasks.insert(tasks.end(), ct.begin(), ct.end());
         st.tasks[cls] = (int64_t)ct.size();
         st.integrals[cls] = st.shell_quartets[cls] * CLASS_NINT[cls];
@@ -563,6 +597,26 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         st.near_threshold = near;
     }
     pr.stats = st;
+    pr.compact_ready = false;
+    if (precision == 1) {
+        std::vector<int2> psh(npair);
+        std::vector<float> pq(npair);
+        for (int i = 0; i < npair; ++i) {
+            psh[i] = make_int2(pr.pairs[i].sa, pr.pairs[i].sb);
+            pq[i] = (float)std::sqrt(std::max(pr.pairs[i].dmax, 0.0));
+        }
+        const size_t nt = std::max<size_t>(tasks.size(), 1);
+        HIPCHK(pr.d_gfirst.ensure(nt * sizeof(int)));
+        HIPCHK(pr.d_glast.ensure(nt * sizeof(int)));
+        HIPCHK(pr.d_pair_shell.ensure(npair * sizeof(int2)));
+        HIPCHK(pr.d_pairQ.ensure(npair * sizeof(float)));
+        HIPCHK(pr.d_Pm.ensure((size_t)ns * ns * sizeof(float)));
+        HIPCHK(hipMemcpyAsync(pr.d_gfirst.p, gfirst_all.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_glast.p, glast_all.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_pair_shell.p, psh.data(), npair * sizeof(int2), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_pairQ.p, pq.data(), npair * sizeof(float), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+    }
     HIPCHK(pr.d_tasks.ensure(std::max<size_t>(tasks.size(), 1) * sizeof(Task)));
     HIPCHK(hipMemcpyAsync(pr.d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, S(ctx)));
     HIPCHK(pr.d_Jb.ensure((size_t)n * n * sizeof(double)));
@@ -581,6 +635,28 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
     return FOCK_OK;
 }
 
+extern "C" fock_status fock_jk_set_timing(fock_ctx *ctx, int32_t enable) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (enable)
+        for (hipEvent_t &e : ctx->tev)
+            if (!e) HIPCHK(hipEventCreate(&e));
+    ctx->timing = enable != 0;
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_jk_get_timings(fock_ctx *ctx, double ms[6]) {
+    if (!ctx || !ms) return FOCK_INVALID_ARG;
+    if (!ctx->timing) return ctx->fail(FOCK_INVALID_ARG, "timing is not enabled");
+    HIPCHK(hipEventSynchronize(ctx->tev[NUM_CLASSES]));
+    for (int c = 0; c < NUM_CLASSES; ++c) {
+        float f = 0.0f;
+        HIPCHK(hipEventElapsedTime(&f, ctx->tev[c], ctx->tev[c + 1]));
+        ms[c] = f;
+    }
+    return FOCK_OK;
+}
+
 extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
     if (!ctx || !stats) return FOCK_INVALID_ARG;
     if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
@@ -590,15 +666,17 @@ extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
 
 namespace {
 
-// Enqueue the six class kernels accumulating into Jb/Kb (zeroed here).
-fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, int mode, double *dense) {
+// Enqueue the six class kernels accumulating into Jb/Kb (zeroed here). fp32 selects the FP32
+// quartet kernels (mixed form); default: the prepared precision.
+fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, int mode, double *dense,
+                       int force_fp64 = 0) {
     Prepared &pr = ctx->jk;
     const int n = pr.n;
     if (mode == MODE_JK) {
         HIPCHK(fill_f64((int64_t)n * n, 0.0, Jb, S(ctx)));
         HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
     }
-    const bool fp32 = (pr.precision == 1) && mode == MODE_JK;
+    const bool fp32 = (pr.precision == 1) && mode == MODE_JK && !force_fp64;
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         JKArgs a{};
         a.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
@@ -613,8 +691,85 @@ fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, 
         a.dthr = pr.dthr;
         a.bt = ctx->bt;
         a.dense = dense;
+        if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[cls], S(ctx)));
         HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
     }
+    if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[NUM_CLASSES], S(ctx)));
+    return FOCK_OK;
+}
+
+fock_status ensure_compact(fock_ctx *ctx) {
+    Prepared &pr = ctx->jk;
+    if (pr.compact_ready) return FOCK_OK;
+    const int64_t S = std::max<int64_t>(pr.max_slots, 64);
+    HIPCHK(pr.d_flag32.ensure(S * sizeof(int)));
+    HIPCHK(pr.d_flag64.ensure(S * sizeof(int)));
+    for (Prepared::Compact *c : {&pr.c32, &pr.c64}) {
+        for (DevBuf *b : {&c->pos, &c->kets, &c->head, &c->tid, &c->kbra, &c->kend, &c->kcost})
+            HIPCHK(b->ensure(S * sizeof(int)));
+        HIPCHK(c->tasks.ensure((S / 64 + 1) * sizeof(Task)));
+        HIPCHK(c->count.ensure(sizeof(int)));
+        c->tmp_bytes = compact_temp_bytes(S);
+        HIPCHK(c->tmp.ensure(std::max<size_t>(c->tmp_bytes, 16)));
+    }
+    pr.compact_ready = true;
+    return FOCK_OK;
+}
+
+CompactBuffers view(Prepared::Compact &c) {
+    return CompactBuffers{c.pos.as<int>(),  c.kets.as<int>(), c.head.as<int>(), c.tid.as<int>(),
+                          c.kbra.as<int>(), c.kend.as<int>(), c.kcost.as<int>(), c.tasks.as<Task>(),
+                          c.count.as<int>(), c.tmp.p, c.tmp_bytes};
+}
+
+// Density-weighted incremental build: J/K of dP over the quartets whose bound
+// Q_ab Q_cd max|dP| reaches tau; bounds >= tau64 run in FP64, the rest in FP32.
+fock_status enqueue_jk_incremental(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, float tau, float tau64,
+                                   bool count_survivors) {
+    Prepared &pr = ctx->jk;
+    const int n = pr.n, ns = (int)pr.shells.size();
+    STCHK(ensure_compact(ctx));
+    HIPCHK(fill_f64((int64_t)n * n, 0.0, Jb, S(ctx)));
+    HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
+    HIPCHK(shell_block_maxabs(ns, pr.d_shell_f0.as<int>(), pr.d_shell_nf.as<int>(), n, dP, pr.d_Pm.as<float>(), S(ctx)));
+    int64_t survivors = 0;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        const int off = pr.task_off[cls], nt = pr.task_off[cls + 1] - off;
+        if (nt == 0) continue;
+        ScreenArgs sa{pr.d_tasks.as<Task>() + off, nt, pr.d_pair_shell.as<int2>(), pr.d_pairQ.as<float>(),
+                      pr.d_Pm.as<float>(), ns, tau, tau64, pr.d_flag32.as<int>(), pr.d_flag64.as<int>()};
+        HIPCHK(screen_flags(sa, S(ctx)));
+        for (int prec = 0; prec < 2; ++prec) {
+            Prepared::Compact &c = prec == 0 ? pr.c32 : pr.c64;
+            const int *flag = prec == 0 ? pr.d_flag32.as<int>() : pr.d_flag64.as<int>();
+            HIPCHK(compact_tasks(pr.d_tasks.as<Task>() + off, nt, pr.d_gfirst.as<int>() + off,
+                                 pr.d_glast.as<int>() + off, flag, view(c), S(ctx)));
+            JKArgs a{};
+            a.tasks = c.tasks.as<Task>();
+            a.ntasks = nt;
+            a.d_ntasks = c.count.as<int>();
+            a.ket_list = c.kets.as<int>();
+            a.nbf = n;
+            a.pairs = pr.d_pairs.as<PairInfo>();
+            a.prims = prec == 0 ? pr.d_prims32.p : pr.d_prims64.p;
+            a.P = dP;
+            a.J = Jb;
+            a.K = Kb;
+            a.D = pr.d_D.as<double>();
+            a.dthr = pr.dthr;
+            a.bt = ctx->bt;
+            HIPCHK(launch_eri_tasks(cls, MODE_JK, a, pr.policy, false, prec == 0, S(ctx)));
+            if (count_survivors) {
+                int last[2];
+                const int64_t Sl = (int64_t)nt * 64;
+                HIPCHK(hipMemcpyAsync(&last[0], c.pos.as<int>() + Sl - 1, sizeof(int), hipMemcpyDeviceToHost, S(ctx)));
+                HIPCHK(hipMemcpyAsync(&last[1], flag + Sl - 1, sizeof(int), hipMemcpyDeviceToHost, S(ctx)));
+                HIPCHK(hipStreamSynchronize(S(ctx)));
+                survivors += (int64_t)last[0] + last[1];
+            }
+        }
+    }
+    pr.last_survivors = survivors;
     return FOCK_OK;
 }
 
@@ -991,6 +1146,27 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     HIPCHK(gwh_guess(n, S_, H_, F_, S(ctx)));
     STCHK(diagonalize(F_));
 
+    // mixed form: phase 0 = FP32 bulk (incremental when requested), phase 1 = FP64 full builds
+    int phase = mixed ? 0 : 1;
+    const bool incremental = mixed && opts.incremental;
+    constexpr int REBUILD = 8;
+    constexpr float INC_TAU = 1e-11f, INC_TAU64 = 1e-5f;
+    constexpr double MIXED_SWITCH = 1e-5;
+    int since_rebuild = 0, stall = 0;
+    double best_err = HUGE_VAL;
+    const bool want_counts = out->history != nullptr;
+    int64_t static_quartets = 0;
+    for (int c = 0; c < NUM_CLASSES; ++c) static_quartets += pr.stats.shell_quartets[c];
+    DevBuf bGacc, bPprev, bPinc;
+    double *Gacc = nullptr, *Pprev = nullptr, *dPinc = nullptr;
+    if (incremental) {
+        HIPCHK(bGacc.ensure(bytes));
+        HIPCHK(bPprev.ensure(bytes));
+        HIPCHK(bPinc.ensure(bytes));
+        Gacc = bGacc.as<double>();
+        Pprev = bPprev.as<double>();
+        dPinc = bPinc.as<double>();
+    }
     std::deque<int> slots;        // ring slots of the stored (error, Fock) pairs, oldest first
     std::deque<std::vector<double>> dots;  // dots[i][j] = <e_i, e_j> for stored i, j (same order)
     int next_slot = 0;
@@ -1009,9 +1185,29 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     for (int iter = 0; iter <= opts.max_iters; ++iter) {
         // G(P), F = H + G (scf/src/lib.rs:177-187)
         HIPCHK(hipEventRecord(ev0, S(ctx)));
-        STCHK(enqueue_jk(ctx, P_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr));
-        HIPCHK(hipEventRecord(ev1, S(ctx)));
-        HIPCHK(fock_from_jk(n, H_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), G_, F_, 0.5, S(ctx)));
+        int64_t evaluated = static_quartets;
+        if (phase == 0 && incremental) {
+            // G(P_k) = G(P_{k-1}) + G(P_k - P_{k-1}); full rebuild every REBUILD iterations
+            if (since_rebuild == 0) {
+                HIPCHK(fill_f64(nn, 0.0, Gacc, S(ctx)));
+                HIPCHK(fill_f64(nn, 0.0, Pprev, S(ctx)));
+            }
+            HIPCHK(sub_f64(nn, P_, Pprev, dPinc, S(ctx)));
+            STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU, INC_TAU64,
+                                         want_counts));
+            HIPCHK(hipEventRecord(ev1, S(ctx)));
+            HIPCHK(accumulate_g(n, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), Gacc, S(ctx)));
+            HIPCHK(copy_f64(nn, P_, Pprev, S(ctx)));
+            HIPCHK(copy_f64(nn, Gacc, G_, S(ctx)));
+            HIPCHK(add_f64(nn, H_, G_, F_, S(ctx)));
+            since_rebuild = (since_rebuild + 1) % REBUILD;
+            if (want_counts) evaluated = pr.last_survivors;
+        } else {
+            // full build: FP32 quartets in the mixed phase, FP64 otherwise (and in the cleanup)
+            STCHK(enqueue_jk(ctx, P_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr, phase == 1));
+            HIPCHK(hipEventRecord(ev1, S(ctx)));
+            HIPCHK(fock_from_jk(n, H_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), G_, F_, 0.5, S(ctx)));
+        }
         // DIIS error e = F P S - S P F = (F P S) - (F P S)^T (F, P, S symmetric) (:189-192)
         const int slot = next_slot;
         next_slot = (next_slot + 1) % 16;
@@ -1074,15 +1270,31 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             h.total_energy = e_elec + e_nuc;
             h.error = err;
             h.fock_build_ms = ms;
-            int64_t q = 0;
-            for (int c = 0; c < NUM_CLASSES; ++c) q += pr.stats.shell_quartets[c];
-            h.quartets = q;
+            h.quartets = evaluated;
         }
         if (debug)
-            std::fprintf(stderr, "[fock] iter %d err %.6e E %.12f build %.3f ms\n", iter, err, e_elec + e_nuc, ms);
+            std::fprintf(stderr, "[fock] iter %d phase %d err %.6e E %.12f build %.3f ms quartets %lld\n", iter, phase,
+                         err, e_elec + e_nuc, ms, (long long)evaluated);
         out->electronic_energy = e_elec;
         out->total_energy = e_elec + e_nuc;
         out->iterations = iter;
+        if (phase == 0) {
+            // mixed phase: hand over to FP64 full builds once near convergence or stalled; the
+            // result always comes from an FP64 build (the cleanup)
+            if (err < best_err * 0.5) {
+                best_err = err;
+                stall = 0;
+            } else {
+                ++stall;
+            }
+            const bool near = err < std::max(10.0 * opts.epsilon, MIXED_SWITCH);
+            if (near || stall >= 3 || iter + 3 >= opts.max_iters) phase = 1;
+            if (!is_normal(err) && !(err < opts.epsilon)) {
+                result = FOCK_DIVERGED;
+                break;
+            }
+            continue;
+        }
         if (err < opts.epsilon || iter == opts.max_iters) {
             result = (err < opts.epsilon) ? FOCK_OK : FOCK_NOT_CONVERGED;
             break;

@@ -28,8 +28,35 @@ struct JKArgs {
     const double *D;  // (xy|xy) of every basis-function pair (reference screen)
     double dthr;
     BoysTables bt;
-    double *dense;  // MODE_DENSE output, n^4
+    double *dense;          // MODE_DENSE output, n^4
+    const int *ket_list;    // compacted tasks: ket pair = ket_list[ket_begin + lane] (null: contiguous)
+    const int *d_ntasks;    // compacted tasks: number of valid tasks on the device (null: ntasks)
 };
+
+// density-weighted screening of the static task list (compact.hip)
+struct ScreenArgs {
+    const Task *tasks;
+    int ntasks;
+    const int2 *pair_shell;  // shells (a, b) of each pair
+    const float *pairQ;      // sqrt(max_component (ab|ab))
+    const float *Pm;         // [nshell][nshell] max |dP| per shell block
+    int nshell;
+    float tau, tau64;
+    int *flag32, *flag64;    // per slot (task * 64 + lane)
+};
+struct CompactBuffers {
+    int *pos, *kets, *head, *tid, *kbra, *kend, *kcost;
+    Task *tasks;
+    int *count;
+    void *tmp;
+    size_t tmp_bytes;
+};
+hipError_t screen_flags(const ScreenArgs &a, hipStream_t s);
+size_t compact_temp_bytes(int64_t slots);
+hipError_t compact_tasks(const Task *tasks, int ntasks, const int *gfirst, const int *glast, const int *flag,
+                         const CompactBuffers &b, hipStream_t s);
+// Gacc += (Jb + Jb^T) - 0.5 (Kb + Kb^T)
+hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc, hipStream_t s);
 
 struct DiagArgs {
     const PairInfo *pairs;

@@ -40,7 +40,7 @@ EXPORTED = [
     "fock_version", "fock_ctx_create", "fock_ctx_destroy", "fock_ctx_last_error", "fock_ctx_set_stream",
     "fock_scf_opts_default", "fock_scf_run", "fock_boys", "fock_eri_dense", "fock_one_electron",
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
-    "fock_sorted_eigs", "fock_diis",
+    "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
 ]
 
 
@@ -120,6 +120,8 @@ def load(path: Optional[str] = None):
         "fock_gemm": (ctypes.c_int, [vp, i32, i32, i32, i32, i32, d, _dp, i32, _dp, i32, d, _dp, i32]),
         "fock_sorted_eigs": (ctypes.c_int, [vp, i32, _dp, _dp, _dp]),
         "fock_diis": (ctypes.c_int, [vp, i32, i64, _dp, _dp, _dp]),
+        "fock_jk_set_timing": (ctypes.c_int, [vp, i32]),
+        "fock_jk_get_timings": (ctypes.c_int, [vp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -214,6 +216,14 @@ class Context:
     def jk_build_device(self, dP: int, dJ: int, dK: int):
         self.check(self.lib.fock_jk_build_device(self.ptr, ctypes.c_void_p(dP), ctypes.c_void_p(dJ),
                                                  ctypes.c_void_p(dK)))
+
+    def jk_set_timing(self, enable: bool = True):
+        self.check(self.lib.fock_jk_set_timing(self.ptr, 1 if enable else 0))
+
+    def jk_timings(self) -> np.ndarray:
+        ms = np.zeros(6)
+        self.check(self.lib.fock_jk_get_timings(self.ptr, _d(ms)))
+        return ms
 
     def jk_stats(self) -> dict:
         st = JKStats()

@@ -210,6 +210,12 @@ fock_status fock_sorted_eigs(fock_ctx *ctx, int32_t n, const double *M, double *
  * reference's None). */
 fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const double *errors, const double *focks, double *out);
 
+/* Per-class kernel timing of J/K builds: when enabled, every class launch is bracketed by HIP
+ * events recorded on the launch stream. fock_jk_get_timings returns the milliseconds of each of
+ * the six class kernels of the most recent build (waits for that build to finish). */
+fock_status fock_jk_set_timing(fock_ctx *ctx, int32_t enable);
+fock_status fock_jk_get_timings(fock_ctx *ctx, double ms[6]);
+
 /* Statistics of the prepared J/K build, per quartet class (ss|ss), (ps|ss), (ps|ps), (pp|ss),
  * (pp|ps), (pp|pp). Counts are of work that survives the screen. */
 typedef struct fock_jk_stats {

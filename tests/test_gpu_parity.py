@@ -198,3 +198,61 @@ def test_scf_o108_accurate_schwarz(ctx, o108):
                 schwarz_threshold=1e-12, ctx=ctx)
     assert r is not None and r.converged
     assert abs(r.total_energy - o108["o108_acc_energy"][0]) < 1e-8, (r.total_energy, o108["o108_acc_energy"][0])
+
+
+# ------------------------------------------------------------------ dense SCF steps
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("shape", [(16, 16, 4), (20, 13, 7), (108, 108, 33), (264, 264, 264)])
+def test_gemm_fp64_mfma(ctx, ta, tb, shape):
+    mm, nn, kk = shape
+    rng = np.random.default_rng(mm * 7 + nn)
+    A = rng.standard_normal((kk, mm) if ta else (mm, kk))
+    B = rng.standard_normal((nn, kk) if tb else (kk, nn))
+    C0 = rng.standard_normal((mm, nn))
+    C = ctx.gemm(A, B, bool(ta), bool(tb), alpha=1.5, beta=-0.5, C=C0)
+    ref = 1.5 * (A.T if ta else A) @ (B.T if tb else B) - 0.5 * C0
+    assert np.abs(C - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()) * kk
+
+
+def test_sorted_eigs(ctx):
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((64, 64))
+    M = M + M.T
+    v, w = ctx.sorted_eigs(M)
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - np.linalg.eigvalsh(M)).max() < 1e-12
+    assert np.abs(M @ v - v * w).max() < 1e-11
+    assert np.abs(v.T @ v - np.eye(64)).max() < 1e-12
+
+
+def test_diis_matches_restatement(ctx):
+    from oracle import refscf
+    rng = np.random.default_rng(9)
+    errs = [rng.standard_normal((7, 7)) * 10.0 ** (-k) for k in range(6)]
+    focks = [rng.standard_normal((7, 7)) for _ in range(6)]
+    got = ctx.diis(errs, focks)
+    want = refscf.diis(errs, focks)
+    assert np.abs(got - want).max() < 1e-9 * np.abs(want).max()
+
+
+# ------------------------------------------------------------------ mixed precision form
+
+def test_mixed_jk_full_build_close_to_fp64(ctx, o108):
+    say("mixed jk o108")
+    fb, _ = m.flatten(m.octane_631g())
+    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12, abi.PRECISION_MIXED)
+    J, K = ctx.jk_build(o108["P_random"])
+    # FP32 quartets, FP64 accumulation: relative agreement at FP32 level
+    assert np.abs(J - o108["o108_acc_J"]).max() < 2e-5 * np.abs(o108["o108_acc_J"]).max()
+    assert np.abs(K - o108["o108_acc_K"]).max() < 2e-5 * np.abs(o108["o108_acc_K"]).max()
+
+
+@pytest.mark.parametrize("incremental", [False, True])
+def test_scf_o108_mixed(ctx, o108, incremental):
+    say("scf o108 mixed incremental=%s" % incremental)
+    from fock_amd.scf import try_scf
+    r = try_scf(m.octane_631g(), 200, 1e-9, 0, policy=abi.POLICY_ACCURATE, screen=abi.SCREEN_SCHWARZ,
+                schwarz_threshold=1e-12, precision=abi.PRECISION_MIXED, incremental=incremental, ctx=ctx)
+    assert r is not None and r.converged
+    assert abs(r.total_energy - o108["o108_acc_energy"][0]) < 1e-6, (r.total_energy, o108["o108_acc_energy"][0])
