@@ -301,4 +301,16 @@ hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc,
     return hipGetLastError();
 }
 
+__global__ void level_shift_kernel(int64_t len, const double *__restrict__ F, const double *__restrict__ Sm,
+                                   const double *__restrict__ SPS, double sigma, double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < len) out[x] = F[x] + sigma * (Sm[x] - 0.5 * SPS[x]);
+}
+hipError_t level_shift_fock(int64_t len, const double *F, const double *Sm, const double *SPS, double sigma,
+                            double *out, hipStream_t s) {
+    hipLaunchKernelGGL(level_shift_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, F, Sm, SPS,
+                       sigma, out);
+    return hipGetLastError();
+}
+
 }  // namespace fock

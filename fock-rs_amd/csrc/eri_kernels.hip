@@ -21,23 +21,15 @@
 
 namespace fock {
 
-template <int LA, int LB, int LC, int LD, int POLICY, int MODE, bool REFSCREEN, typename Real>
-__global__ __launch_bounds__(256) void eri_task_kernel(JKArgs args) {
+// Contracted integrals of one lane's quartet (bra pair of the task, ket pair of the lane), with
+// the reference's pair screen applied when REFSCREEN; inactive lanes return zeros.
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
+__device__ __forceinline__ void lane_integrals(const JKArgs &args, const PairInfo &bra, const PairInfo &ket,
+                                               bool active,
+                                               Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)]) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wave >= args.ntasks) return;  // wave-uniform
-    if (args.d_ntasks && wave >= *args.d_ntasks) return;
-    const Task task = args.tasks[wave];
-    const bool active = lane < task.ket_count;
-    const PairInfo bra = args.pairs[task.bra];
-    const int slot = task.ket_begin + (active ? lane : 0);
-    const int ket_idx = args.ket_list ? args.ket_list[slot] : slot;
-    const PairInfo ket = args.pairs[ket_idx];
     const PrimPair<Real> *__restrict__ pp = reinterpret_cast<const PrimPair<Real> *>(args.prims);
-
-    Real I[NAB][NCD];
 #pragma unroll
     for (int x = 0; x < NAB; ++x)
 #pragma unroll
@@ -79,6 +71,26 @@ __global__ __launch_bounds__(256) void eri_task_kernel(JKArgs args) {
 #pragma unroll
             for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
     }
+}
+
+template <int LA, int LB, int LC, int LD, int POLICY, int MODE, bool REFSCREEN, typename Real>
+__global__ __launch_bounds__(256) void eri_task_kernel(JKArgs args) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB, NCD = NC * ND;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wave >= args.ntasks) return;  // wave-uniform
+    if (args.d_ntasks && wave >= *args.d_ntasks) return;
+    const Task task = args.tasks[wave];
+    const bool active = lane < task.ket_count;
+    const PairInfo bra = args.pairs[task.bra];
+    const int slot = task.ket_begin + (active ? lane : 0);
+    const int ket_idx = args.ket_list ? args.ket_list[slot] : slot;
+    const PairInfo ket = args.pairs[ket_idx];
+    Real I[NAB][NCD];
+    lane_integrals<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, ket, active, I);
+    const int n = args.nbf;
+    const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
 
     if constexpr (MODE == MODE_DENSE) {
         if (!active) return;
@@ -224,6 +236,139 @@ __global__ __launch_bounds__(256) void eri_diag_kernel(DiagArgs args) {
         }
 }
 
+// Blocked J/K digestion: one workgroup per block of tasks that share a bra pair (a, b). Every K
+// contribution of such a task lands in row a or row b (K[a][c], K[a][d], K[b][c], K[b][d]), so the
+// workgroup accumulates those rows in LDS with LDS atomics and flushes each nonzero entry once;
+// J[a][b] is reduced over the workgroup; only J[c][d] is a per-lane global atomic.
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
+__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB, NCD = NC * ND;
+    extern __shared__ double lds[];
+    const int n = args.nbf;
+    double *Ka = lds;                 // [NA][n]
+    double *Kb = lds + NA * n;        // [NB][n]
+    double *red = Kb + NB * n;        // [4][NAB]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Block blk = args.blocks[blockIdx.x];
+    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
+    __syncthreads();
+    const PairInfo bra = args.pairs[blk.bra];
+    const int fa = bra.fa, fb = bra.fb;
+    const size_t N = (size_t)n;
+    const double *__restrict__ P = args.P;
+    double *__restrict__ J = args.J;
+    Real pab[NAB];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)P[(fa + a) * N + (fb + b)];
+    double jab[NAB];
+#pragma unroll
+    for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
+
+    for (int t = wave; t < blk.ntasks; t += 4) {
+        const Task task = args.tasks[blk.first_task + t];
+        const bool active = lane < task.ket_count;
+        const int ket_idx = task.ket_begin + (active ? lane : 0);
+        const PairInfo ket = args.pairs[ket_idx];
+        Real I[NAB][NCD];
+        lane_integrals<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, ket, active, I);
+        const int fc = ket.fa, fd = ket.fb;
+        Real w = Real(1);
+        if (fa == fb) w *= Real(0.5);
+        if (fc == fd) w *= Real(0.5);
+        if (task.bra == ket_idx) w *= Real(0.5);
+        // J[a][b] partial (lane-local; reduced after the loop)
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                Real s = Real(0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fc + c) * N + (fd + d)];
+                jab[a * NB + b] += (double)(s * w);
+            }
+        if (!active) continue;
+        // J[c][d]: global
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                Real s = Real(0);
+#pragma unroll
+                for (int x = 0; x < NAB; ++x) s += I[x][c * ND + d] * pab[x];
+                unsafeAtomicAdd(&J[(fc + c) * N + (fd + d)], 2.0 * (double)(s * w));
+            }
+        // K rows a and b: LDS
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {  // K[a][c] += sum_bd v P[b][d]
+                Real s = Real(0);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fd + d)];
+                atomicAdd(&Ka[a * n + fc + c], (double)(s * w));
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {  // K[a][d] += sum_bc v P[b][c]
+                Real s = Real(0);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fc + c)];
+                atomicAdd(&Ka[a * n + fd + d], (double)(s * w));
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {  // K[b][c] += sum_ad v P[a][d]
+                Real s = Real(0);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fd + d)];
+                atomicAdd(&Kb[b * n + fc + c], (double)(s * w));
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {  // K[b][d] += sum_ac v P[a][c]
+                Real s = Real(0);
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fc + c)];
+                atomicAdd(&Kb[b * n + fd + d], (double)(s * w));
+            }
+        }
+    }
+    // J[a][b]: wave reduction, then across the four waves
+#pragma unroll
+    for (int x = 0; x < NAB; ++x) {
+        const double v = wave_sum(jab[x]);
+        if (lane == 0) red[wave * NAB + x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NAB) {
+        const double v = red[threadIdx.x] + red[NAB + threadIdx.x] + red[2 * NAB + threadIdx.x] + red[3 * NAB + threadIdx.x];
+        if (v != 0.0) unsafeAtomicAdd(&J[(fa + threadIdx.x / NB) * N + (fb + threadIdx.x % NB)], 2.0 * v);
+    }
+    // flush the K rows (nonzero entries only)
+    double *__restrict__ K = args.K;
+    for (int e = threadIdx.x; e < NA * n; e += blockDim.x) {
+        const double v = Ka[e];
+        if (v != 0.0) unsafeAtomicAdd(&K[(fa + e / n) * N + (e % n)], v);
+    }
+    for (int e = threadIdx.x; e < NB * n; e += blockDim.x) {
+        const double v = Kb[e];
+        if (v != 0.0) unsafeAtomicAdd(&K[(fb + e / n) * N + (e % n)], v);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 template <int LA, int LB, int LC, int LD, int POLICY, int MODE, bool REFSCREEN, typename Real>
@@ -251,6 +396,42 @@ static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool ref
                          : launch_one<LA, LB, LC, LD, 0, MODE, false, double>(a, s);
     return refscreen ? launch_one<LA, LB, LC, LD, 1, MODE, true, double>(a, s)
                      : launch_one<LA, LB, LC, LD, 1, MODE, false, double>(a, s);
+}
+
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
+static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
+    if (a.nblocks <= 0) return hipSuccess;
+    const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
+                       4 * ncart(LA) * ncart(LB) * sizeof(double);
+    hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real>), dim3(a.nblocks), dim3(256), lds, s,
+                       a);
+    return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+static hipError_t dispatch_block_policy(const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s) {
+    if (fp32) {
+        if (policy != 1) return hipErrorInvalidValue;
+        return refscreen ? launch_block<LA, LB, LC, LD, 1, true, float>(a, s)
+                         : launch_block<LA, LB, LC, LD, 1, false, float>(a, s);
+    }
+    if (policy == 0)
+        return refscreen ? launch_block<LA, LB, LC, LD, 0, true, double>(a, s)
+                         : launch_block<LA, LB, LC, LD, 0, false, double>(a, s);
+    return refscreen ? launch_block<LA, LB, LC, LD, 1, true, double>(a, s)
+                     : launch_block<LA, LB, LC, LD, 1, false, double>(a, s);
+}
+
+hipError_t launch_eri_blocks(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s) {
+    switch (cls) {
+        case 0: return dispatch_block_policy<0, 0, 0, 0>(a, policy, refscreen, fp32, s);
+        case 1: return dispatch_block_policy<1, 0, 0, 0>(a, policy, refscreen, fp32, s);
+        case 2: return dispatch_block_policy<1, 0, 1, 0>(a, policy, refscreen, fp32, s);
+        case 3: return dispatch_block_policy<1, 1, 0, 0>(a, policy, refscreen, fp32, s);
+        case 4: return dispatch_block_policy<1, 1, 1, 0>(a, policy, refscreen, fp32, s);
+        case 5: return dispatch_block_policy<1, 1, 1, 1>(a, policy, refscreen, fp32, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <int MODE>

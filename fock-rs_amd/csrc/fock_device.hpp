@@ -73,6 +73,11 @@ struct Task {
     int bra, ket_begin, ket_count, flags;
 };
 
+// One workgroup of the blocked digestion: tasks [first_task, first_task + ntasks) of one bra pair.
+struct Block {
+    int first_task, ntasks, bra, cost;
+};
+
 // ------------------------------------------------------------------ Boys
 
 constexpr int BOYS_GRID_N = 451;        // T_i = 0.1 i, i = 0..450

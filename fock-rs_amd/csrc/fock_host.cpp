@@ -33,6 +33,9 @@ const double REF_LUT_HOST[200] = {
 #include "ref_boys_lut.inc"
 };
 
+// tasks (64 kets each) per workgroup of the blocked digestion
+constexpr int BLOCK_TASKS = 8;
+
 // f_prim of the canonical McMurchie-Davidson count per quartet class (see fock_jk_stats)
 constexpr double F_PRIM[NUM_CLASSES] = {38, 85, 266, 298, 977, 3833};
 constexpr int CLASS_NINT[NUM_CLASSES] = {1, 3, 9, 9, 27, 81};
@@ -77,8 +80,9 @@ struct Prepared {
     double tau = 1e-12, dthr = 0.0;
     std::vector<Shell> shells;
     std::vector<PairHost> pairs;  // in task (sorted) order
-    DevBuf d_pairs, d_prims64, d_prims32, d_D, d_tasks;
+    DevBuf d_pairs, d_prims64, d_prims32, d_D, d_tasks, d_blocks;
     int task_off[NUM_CLASSES + 1] = {0};
+    int block_off[NUM_CLASSES + 1] = {0};
     fock_jk_stats stats{};
     DevBuf d_Jb, d_Kb, d_P, d_J, d_K;
     // shell metadata on device (density-weighted screening)
@@ -331,6 +335,11 @@ extern "C" void fock_scf_opts_default(fock_scf_opts *o) {
     o->schwarz_threshold = 1e-12;
     o->precision = FOCK_PRECISION_FP64;
     o->incremental = 0;
+    o->initial_density = nullptr;
+    o->diis_start = 5;
+    o->diis_window = 16;
+    o->level_shift = 0.0;
+    o->level_shift_until = 0.0;
 }
 
 // ====================================================================== Boys
@@ -508,6 +517,7 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         return di * dj > pr.dthr;
     };
     std::vector<Task> tasks;
+    std::vector<Block> blocks_all;
     std::vector<int> gfirst_all, glast_all;
     fock_jk_stats st{};
     st.n_basis = n;
@@ -558,18 +568,26 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
                 if (hi > lo) ++ngroup;
             }
         }
-        // most expensive tasks first (load balance); stable for determinism. The tasks of one
-        // (bra, bucket) group share their cost and stay adjacent.
-        std::vector<int> perm(ct.size());
-        std::iota(perm.begin(), perm.end(), 0);
-        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return ct[a].flags > ct[b].flags; });
-        std::vector<Task> sorted(ct.size());
-        std::vector<int> sg(ct.size());
-        for (size_t q = 0; q < perm.size(); ++q) {
-            sorted[q] = ct[perm[q]];
-            sg[q] = cgid[perm[q]];
+        // tasks stay bra-major (the tasks of one bra pair, and of one (bra, bucket) group, are
+        // adjacent); blocks of up to BLOCK_TASKS tasks of one bra pair feed the blocked digestion,
+        // most expensive blocks first (load balance; stable for determinism)
+        const std::vector<int> &sg = cgid;
+        {
+            std::vector<Block> cb;
+            for (size_t q = 0; q < ct.size();) {
+                size_t r = q;
+                int cost = 0;
+                while (r < ct.size() && ct[r].bra == ct[q].bra && r - q < (size_t)BLOCK_TASKS) {
+                    cost += ct[r].flags * ct[r].ket_count;
+                    ++r;
+                }
+                cb.push_back(Block{(int)q, (int)(r - q), ct[q].bra, cost});
+                q = r;
+            }
+            std::stable_sort(cb.begin(), cb.end(), [](const Block &a, const Block &b) { return a.cost > b.cost; });
+            pr.block_off[cls] = (int)blocks_all.size();
+            blocks_all.insert(blocks_all.end(), cb.begin(), cb.end());
         }
-        ct.swap(sorted);
         std::vector<int> gf(ct.size()), gl(ct.size());
         for (size_t q = 0; q < ct.size(); ++q) gf[q] = (q > 0 && sg[q] == sg[q - 1]) ? gf[q - 1] : (int)q;
         for (size_t q = ct.size(); q-- > 0;) gl[q] = (q + 1 < ct.size() && sg[q] == sg[q + 1]) ? gl[q + 1] : (int)q;
@@ -582,6 +600,10 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         st.algorithmic_flop[cls] = (double)st.primitive_quartets[cls] * F_PRIM[cls] + 12.0 * (double)st.integrals[cls];
     }
     pr.task_off[NUM_CLASSES] = (int)tasks.size();
+    pr.block_off[NUM_CLASSES] = (int)blocks_all.size();
+    HIPCHK(pr.d_blocks.ensure(std::max<size_t>(blocks_all.size(), 1) * sizeof(Block)));
+    HIPCHK(hipMemcpyAsync(pr.d_blocks.p, blocks_all.data(), blocks_all.size() * sizeof(Block), hipMemcpyHostToDevice,
+                          S(ctx)));
     if (screen == FOCK_SCREEN_REF_BF_PAIR) {
         // basis-function pair products within 1e-12 of the reference's threshold (estimate)
         std::vector<double> dv;
@@ -691,8 +713,13 @@ fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, 
         a.dthr = pr.dthr;
         a.bt = ctx->bt;
         a.dense = dense;
+        a.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
+        a.nblocks = pr.block_off[cls + 1] - pr.block_off[cls];
         if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[cls], S(ctx)));
-        HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
+        if (mode == MODE_JK && n <= BLOCKED_MAX_NBF)
+            HIPCHK(launch_eri_blocks(cls, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
+        else
+            HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
     }
     if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[NUM_CLASSES], S(ctx)));
     return FOCK_OK;
@@ -1068,6 +1095,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         return ctx->fail(FOCK_INVALID_ARG, "no nuclei");
     if (opts.incremental && opts.precision != FOCK_PRECISION_MIXED)
         return ctx->fail(FOCK_INVALID_ARG, "incremental builds are part of the mixed-precision form");
+    if (opts.diis_window < 1 || opts.diis_window > 16 || opts.diis_start < 1)
+        return ctx->fail(FOCK_INVALID_ARG, "DIIS window must be 1..16 and start >= 1");
+    if (!(opts.level_shift >= 0.0)) return ctx->fail(FOCK_INVALID_ARG, "negative level shift");
     const auto t_start = std::chrono::steady_clock::now();
     const bool debug = std::getenv("FOCK_DEBUG") != nullptr;
     HIPCHK(hipSetDevice(ctx->device));
@@ -1142,9 +1172,13 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         return FOCK_OK;
     };
 
-    // GWH guess (scf/src/lib.rs:150-162)
-    HIPCHK(gwh_guess(n, S_, H_, F_, S(ctx)));
-    STCHK(diagonalize(F_));
+    if (opts.initial_density) {
+        HIPCHK(hipMemcpyAsync(P_, opts.initial_density, bytes, hipMemcpyHostToDevice, S(ctx)));
+    } else {
+        // GWH guess (scf/src/lib.rs:150-162)
+        HIPCHK(gwh_guess(n, S_, H_, F_, S(ctx)));
+        STCHK(diagonalize(F_));
+    }
 
     // mixed form: phase 0 = FP32 bulk (incremental when requested), phase 1 = FP64 full builds
     int phase = mixed ? 0 : 1;
@@ -1218,8 +1252,8 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         HIPCHK(fill_f64(1, 0.0, scal, S(ctx)));
         HIPCHK(antisym_maxabs(n, T2, e_slot, scal, S(ctx)));
         HIPCHK(copy_f64(nn, F_, f_slot, S(ctx)));
-        // window of 16 (:194-200)
-        if ((int)slots.size() == 16) {
+        // window of diis_window, the reference's 16 (:194-200)
+        if ((int)slots.size() == opts.diis_window) {
             slots.pop_front();
             dots.pop_front();
             for (auto &row : dots) row.erase(row.begin());
@@ -1241,9 +1275,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         const double err = out->last_error;
         for (int i = 0; i < m - 1; ++i) dots[i].push_back(newdots[i]);
         dots.push_back(newdots);
-        // extrapolate from 5 stored matrices on (:202-207)
+        // extrapolate from diis_start stored matrices on, the reference's 5 (:202-207)
         const double *Fuse = f_slot;
-        if (m >= 5) {
+        if (m >= opts.diis_start) {
             std::vector<double> Bm((size_t)m * m), c;
             for (int i = 0; i < m; ++i)
                 for (int j = 0; j < m; ++j) Bm[(size_t)i * m + j] = dots[i][j];
@@ -1255,6 +1289,13 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 HIPCHK(hipStreamSynchronize(S(ctx)));  // c, sl are host temporaries
                 Fuse = Fx_;
             }
+        }
+        if (opts.level_shift > 0.0 && err > opts.level_shift_until) {
+            // virtual orbitals raised by sigma: F + sigma (S - S P S / 2)
+            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, S_, n, P_, n, 0.0, T1, n, S(ctx)));
+            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, T1, n, S_, n, 0.0, T2, n, S(ctx)));
+            HIPCHK(level_shift_fock(nn, Fuse, S_, T2, opts.level_shift, Fx_, S(ctx)));
+            Fuse = Fx_;
         }
         // F' = X^T F X, eigen, C = X C', P_new (:210-216)
         STCHK(diagonalize(Fuse));

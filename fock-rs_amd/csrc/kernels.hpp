@@ -31,7 +31,11 @@ struct JKArgs {
     double *dense;          // MODE_DENSE output, n^4
     const int *ket_list;    // compacted tasks: ket pair = ket_list[ket_begin + lane] (null: contiguous)
     const int *d_ntasks;    // compacted tasks: number of valid tasks on the device (null: ntasks)
+    const Block *blocks;    // blocked digestion: groups of tasks sharing one bra pair
+    int nblocks;
 };
+// largest basis for the blocked digestion: (3 + 3) rows of n doubles in LDS
+constexpr int BLOCKED_MAX_NBF = 1300;
 
 // density-weighted screening of the static task list (compact.hip)
 struct ScreenArgs {
@@ -57,6 +61,9 @@ hipError_t compact_tasks(const Task *tasks, int ntasks, const int *gfirst, const
                          const CompactBuffers &b, hipStream_t s);
 // Gacc += (Jb + Jb^T) - 0.5 (Kb + Kb^T)
 hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc, hipStream_t s);
+// out = F + sigma (S - 0.5 SPS)
+hipError_t level_shift_fock(int64_t len, const double *F, const double *S, const double *SPS, double sigma,
+                            double *out, hipStream_t s);
 
 struct DiagArgs {
     const PairInfo *pairs;
@@ -69,6 +76,7 @@ struct DiagArgs {
 hipError_t launch_eri_tasks(int cls, int mode, const JKArgs &a, int policy, bool refscreen, bool fp32,
                             hipStream_t s);
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);
+hipError_t launch_eri_blocks(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s);
 
 // one_electron.hip
 struct OneElArgs {

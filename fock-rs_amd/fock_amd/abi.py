@@ -57,7 +57,9 @@ class ScfOpts(ctypes.Structure):
     _fields_ = [("max_iters", ctypes.c_int32), ("epsilon", ctypes.c_double), ("molecule_charge", ctypes.c_int32),
                 ("integral_policy", ctypes.c_int32), ("screen", ctypes.c_int32),
                 ("schwarz_threshold", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("incremental", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("incremental", ctypes.c_int32), ("initial_density", _dp), ("diis_start", ctypes.c_int32),
+                ("diis_window", ctypes.c_int32), ("level_shift", ctypes.c_double),
+                ("level_shift_until", ctypes.c_double), ("reserved", ctypes.c_int32 * 4)]
 
 
 class ScfIter(ctypes.Structure):

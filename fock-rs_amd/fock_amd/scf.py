@@ -91,16 +91,26 @@ class HartreeFockResult:
 def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_charge: int = 0,
             policy: int = abi.POLICY_REF_COMPAT, screen: int = abi.SCREEN_REF_BF_PAIR,
             schwarz_threshold: float = 1e-12, precision: int = abi.PRECISION_FP64,
-            incremental: bool = False, ctx: Optional[abi.Context] = None) -> Optional[HartreeFockResult]:
+            incremental: bool = False, ctx: Optional[abi.Context] = None,
+            initial_density: Optional[np.ndarray] = None, diis_start: int = 5, diis_window: int = 16,
+            level_shift: float = 0.0, level_shift_until: float = 0.0) -> Optional[HartreeFockResult]:
     """SelfConsistentField::try_scf (scf/src/lib.rs:73-258) on the GPU.
 
-    Defaults reproduce the reference's own arithmetic (REF_COMPAT integrals, its 1e-6
-    basis-function-pair screen, FP64)."""
+    Defaults reproduce the reference's own arithmetic and schedule (REF_COMPAT integrals, its
+    1e-6 basis-function-pair screen, FP64, GWH guess, DIIS from 5 stored matrices in a window
+    of 16)."""
     atoms = list(atoms)
     fb, fn = flatten(atoms)
     opts = abi.default_opts(max_iters=max_iters, epsilon=epsilon, molecule_charge=molecule_charge,
                             integral_policy=policy, screen=screen, schwarz_threshold=schwarz_threshold,
-                            precision=precision, incremental=1 if incremental else 0)
+                            precision=precision, incremental=1 if incremental else 0,
+                            diis_start=diis_start, diis_window=diis_window, level_shift=level_shift,
+                            level_shift_until=level_shift_until)
+    if initial_density is not None:
+        P0 = np.ascontiguousarray(initial_density, dtype=np.float64)
+        if P0.shape != (fb.n, fb.n):
+            raise ValueError("initial density must be n x n")
+        opts.initial_density = abi._d(P0)
     ctx = ctx or context()
     st, r = ctx.scf_run(fb, fn, opts)
     if st == abi.FOCK_DIVERGED:

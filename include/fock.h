@@ -103,7 +103,18 @@ typedef struct fock_scf_opts {
     /* MIXED only: build G(P_k) = G(P_{k-1}) + G(P_k - P_{k-1}) with density-weighted screening,
      * then one final full FP64 build at the converged density. */
     int32_t incremental;
-    int32_t reserved[6];
+    /* Optional starting density (n x n, symmetric) replacing the GWH guess of
+     * scf/src/lib.rs:148-162; NULL for the reference's guess. */
+    const double *initial_density;
+    /* DIIS schedule: extrapolate once diis_start (error, Fock) pairs are stored, keep the last
+     * diis_window (<= 16). The reference uses 5 and 16 (scf/src/lib.rs:197-207). */
+    int32_t diis_start;
+    int32_t diis_window;
+    /* Level shift sigma added to the extrapolated Fock matrix as sigma (S - S P S / 2) while
+     * max|FDS - SDF| > level_shift_until (0: none, as the reference). */
+    double level_shift;
+    double level_shift_until;
+    int32_t reserved[4];
 } fock_scf_opts;
 
 /* Per-iteration record (optional output). */

@@ -154,12 +154,15 @@ def density(C: np.ndarray, n_electrons: int) -> np.ndarray:
 
 
 def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, accurate: bool = False,
-            screen: bool = True, integrals=None) -> Optional[ScfResult]:
+            screen: bool = True, integrals=None, P0=None, diis_start: int = 5, diis_window: int = 16,
+            level_shift: float = 0.0, level_shift_until: float = 0.0) -> Optional[ScfResult]:
     """scf/src/lib.rs:73-258 on the flat basis `fb` and nuclei `fn`.
 
     accurate=False: the reference's own integrals (refcpu.c, LUT Boys, x-branch as written,
     1e-6 pair screen when `screen`). accurate=True: oracle64.c, unscreened.
-    `integrals` may pass precomputed (S, T, V, packed) to skip their evaluation.
+    `integrals` may pass precomputed (S, T, V, packed) to skip their evaluation; `P0` replaces
+    the GWH guess by a given density. diis_start / diis_window / level_shift(_until) generalise
+    the reference's constants (5, 16, no shift) exactly as the C ABI options do.
     """
     n = fb.n
     n_electrons = int(round(float(np.sum(fn.charges)))) - molecule_charge
@@ -175,11 +178,14 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
     D = U.T @ (S @ U)
     X = U @ (np.diag(1.0 / np.sqrt(np.diag(D))) @ U.T)
     # GWH guess (scf/src/lib.rs:150-162)
-    hd = np.diag(H)
-    Heht = 0.875 * S * (hd[:, None] + hd[None, :])
-    Cp, _ = sorted_eigs(X.T @ (Heht @ X))
-    C = X @ Cp
-    P = density(C, n_electrons)
+    if P0 is not None:  # restart from a given density instead of the GWH guess
+        P = np.array(P0, dtype=np.float64)
+    else:
+        hd = np.diag(H)
+        Heht = 0.875 * S * (hd[:, None] + hd[None, :])
+        Cp, _ = sorted_eigs(X.T @ (Heht @ X))
+        C = X @ Cp
+        P = density(C, n_electrons)
     gbuild = GBuilder(n, packed)
     focks, errs = deque(), deque()
     err_hist, e_hist = [], []
@@ -190,15 +196,17 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
         err = float(np.max(np.abs(E)))
         errs.append(E)
         focks.append(F)
-        if len(errs) > 16:
+        if len(errs) > diis_window:
             errs.popleft()
             focks.popleft()
-        if len(focks) < 5:
+        if len(focks) < diis_start:
             Fx = focks[-1]
         else:
             Fx = diis(errs, focks)
             if Fx is None:
                 Fx = focks[-1]
+        if level_shift > 0.0 and err > level_shift_until:
+            Fx = Fx + level_shift * (S - 0.5 * (S @ P) @ S)
         Cp, eps = sorted_eigs(X.T @ (Fx @ X))
         C = X @ Cp
         P = density(C, n_electrons)

@@ -35,13 +35,14 @@ def test_default_options_are_the_reference_constants():
     assert o.max_iters == 100 and o.epsilon == 1e-6 and o.molecule_charge == 0
     assert o.integral_policy == abi.POLICY_REF_COMPAT and o.screen == abi.SCREEN_REF_BF_PAIR
     assert o.precision == abi.PRECISION_FP64 and o.incremental == 0
+    assert o.diis_start == 5 and o.diis_window == 16 and o.level_shift == 0.0  # scf/src/lib.rs:197-207
 
 
 def test_struct_sizes_match_header_layout():
     # offsets of the C structs (x86-64 SysV): keeps the ctypes mirror honest
     assert ctypes.sizeof(abi.BasisDesc) == 48
     assert ctypes.sizeof(abi.NucleiDesc) == 24
-    assert ctypes.sizeof(abi.ScfOpts) == 72
+    assert ctypes.sizeof(abi.ScfOpts) == 96
     assert ctypes.sizeof(abi.ScfIter) == 32
     assert ctypes.sizeof(abi.JKStats) == 12 + 4 + 6 * 8 * 5 + 8
 
