@@ -213,7 +213,7 @@ def main():
     achieved = flop / (mixed["class_ms"][dom] * 1e-3) / 1e12
     total_flop = sum(mixed["stats"]["algorithmic_flop"])
     roofline = {
-        "bound": "valu", "kernel": "eri_task_kernel %s fp32" % CLASS_NAMES[dom],
+        "bound": "valu", "kernel": "eri_block_kernel %s fp32" % CLASS_NAMES[dom],
         "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
         "traffic": load_traffic("mixed_" + CLASS_NAMES[dom]),
         "whole_build_tflops": total_flop / (mixed["ms"] * 1e-3) / 1e12,

@@ -73,6 +73,54 @@ __device__ __forceinline__ void lane_integrals(const JKArgs &args, const PairInf
     }
 }
 
+// Same integrals with the Boys slab in LDS and wave-uniform bra primitive pairs: the ket primitive
+// pair is loaded once per lane and reused for every bra primitive.
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
+__device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const PairInfo &bra,
+                                                   const PrimPair<Real> *__restrict__ bra_pp, const PairInfo &ket,
+                                                   bool active,
+                                                   const Real *slice,
+                                                   Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)]) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NAB = NA * NB, NCD = NC * ND;
+    const PrimPair<Real> *__restrict__ pp = reinterpret_cast<const PrimPair<Real> *>(args.prims);
+#pragma unroll
+    for (int x = 0; x < NAB; ++x)
+#pragma unroll
+        for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
+    for (int k = 0; k < ket.K; ++k) {
+        const PrimPair<Real> kq = pp[ket.pp_start + k];
+        for (int i = 0; i < bra.K; ++i) {
+            const PrimPair<Real> b = bra_pp[i];
+            accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+        }
+    }
+    const int n = args.nbf;
+    const int fa = bra.fa, fb = bra.fb, fc = ket.fa, fd = ket.fb;
+    if constexpr (REFSCREEN) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const double dab = args.D[(size_t)(fa + a) * n + (fb + b)];
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) {
+                        const double dcd = args.D[(size_t)(fc + c) * n + (fd + d)];
+                        const bool diag = (fa + a == fc + c && fb + b == fd + d) || (fa + a == fd + d && fb + b == fc + c);
+                        if (!diag && !(dab * dcd > args.dthr)) I[a * NB + b][c * ND + d] = Real(0);
+                    }
+            }
+    }
+    if (!active) {
+#pragma unroll
+        for (int x = 0; x < NAB; ++x)
+#pragma unroll
+            for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
+    }
+}
+
 template <int LA, int LB, int LC, int LD, int POLICY, int MODE, bool REFSCREEN, typename Real>
 __global__ __launch_bounds__(256) void eri_task_kernel(JKArgs args) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -241,19 +289,35 @@ __global__ __launch_bounds__(256) void eri_diag_kernel(DiagArgs args) {
 // workgroup accumulates those rows in LDS with LDS atomics and flushes each nonzero entry once;
 // J[a][b] is reduced over the workgroup; only J[c][d] is a per-lane global atomic.
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
-__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
+__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimPair<Real> *__restrict__ prims) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
+    constexpr int L = LA + LB + LC + LD;
+    constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
     extern __shared__ double lds[];
     const int n = args.nbf;
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
     double *red = Kb + NB * n;        // [4][NAB]
+    Real *slice = reinterpret_cast<Real *>(red + 4 * NAB);  // [BOYS_GRID_N][COLS]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const Block blk = args.blocks[blockIdx.x];
-    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
-    __syncthreads();
     const PairInfo bra = args.pairs[blk.bra];
+    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
+    // the block's bra primitive pairs are read with wave-uniform addresses from a read-only
+    // (no-alias) argument, which lets them go through the scalar cache
+    const PrimPair<Real> *__restrict__ bra_pp = prims + bra.pp_start;
+    {
+        // this class's Boys slab, staged once per workgroup
+        if constexpr (POLICY == 1) {
+            const Real *slab;
+            if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
+            else slab = reinterpret_cast<const Real *>(args.bt.pd32);
+            slab += (size_t)L * BOYS_GRID_N * COLS;
+            for (int e = threadIdx.x; e < BOYS_GRID_N * COLS; e += blockDim.x) slice[e] = slab[e];
+        }
+    }
+    __syncthreads();
     const int fa = bra.fa, fb = bra.fb;
     const size_t N = (size_t)n;
     const double *__restrict__ P = args.P;
@@ -269,11 +333,25 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
 
     for (int t = wave; t < blk.ntasks; t += 4) {
         const Task task = args.tasks[blk.first_task + t];
-        const bool active = lane < task.ket_count;
+        bool active = lane < task.ket_count;
         const int ket_idx = task.ket_begin + (active ? lane : 0);
+        if (args.pair_shell) {
+            // density-weighted screen of this lane's quartet; tasks without a surviving lane are
+            // skipped by the whole wavefront
+            const int2 ab = args.pair_shell[blk.bra], cd = args.pair_shell[ket_idx];
+            const float *Pm = args.Pm;
+            const int ns = args.nshell;
+            float pm = 2.0f * fmaxf(Pm[ab.x * ns + ab.y], Pm[cd.x * ns + cd.y]);
+            pm = fmaxf(pm, fmaxf(Pm[ab.x * ns + cd.x], Pm[ab.x * ns + cd.y]));
+            pm = fmaxf(pm, fmaxf(Pm[ab.y * ns + cd.x], Pm[ab.y * ns + cd.y]));
+            const float bound = args.pairQ[blk.bra] * args.pairQ[ket_idx] * pm;
+            active = active && bound >= args.bound_lo && bound < args.bound_hi;
+            if (__ballot(active) == 0) continue;  // wave-uniform
+        }
         const PairInfo ket = args.pairs[ket_idx];
         Real I[NAB][NCD];
-        lane_integrals<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, ket, active, I);
+        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, ket, active,
+                                                                    POLICY == 1 ? slice : nullptr, I);
         const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
         if (fa == fb) w *= Real(0.5);
@@ -401,10 +479,12 @@ static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool ref
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
 static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
+    constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
     const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
-                       4 * ncart(LA) * ncart(LB) * sizeof(double);
+                       4 * ncart(LA) * ncart(LB) * sizeof(double) +
+                       (POLICY == 1 ? (size_t)BOYS_GRID_N * COLS * sizeof(Real) : 0);
     hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real>), dim3(a.nblocks), dim3(256), lds, s,
-                       a);
+                       a, reinterpret_cast<const PrimPair<Real> *>(a.prims));
     return hipGetLastError();
 }
 

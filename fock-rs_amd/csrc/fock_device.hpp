@@ -87,10 +87,19 @@ constexpr int BOYS_ORDERS = 20;         // tabulated orders m = 0..19 per grid p
 constexpr int BOYS_TAYLOR64 = 8;        // terms (k = 0..7): |error| < 0.05^8/8! F_n ~ 1e-15 F_n
 constexpr int BOYS_TAYLOR32 = 5;        // terms (k = 0..4): < 0.05^5/5! ~ 3e-9 relative
 
+// Per-order Taylor tables with the coefficients already divided by k!: for a top order L,
+// row i holds F_{L+k}(T_i) / k! for k < TERMS followed by exp(-T_i) (PD_COLS columns), so the
+// expansion is a plain Horner chain of FMAs. One [BOYS_GRID_N][PD_COLS] slab per L <= PD_LMAX.
+constexpr int BOYS_PD_LMAX = 4;
+constexpr int BOYS_PD_COLS64 = BOYS_TAYLOR64 + 1;
+constexpr int BOYS_PD_COLS32 = BOYS_TAYLOR32 + 1;
+
 struct BoysTables {
     const double *lut_ref;  // the reference's 200-entry F_10 table
     const double *tab64;    // [BOYS_GRID_N][BOYS_ORDERS]
     const float *tab32;     // [BOYS_GRID_N][BOYS_ORDERS]
+    const double *pd64;     // [BOYS_PD_LMAX + 1][BOYS_GRID_N][BOYS_PD_COLS64]
+    const float *pd32;      // [BOYS_PD_LMAX + 1][BOYS_GRID_N][BOYS_PD_COLS32]
 };
 
 __device__ __forceinline__ double rcp_exact(double x) { return 1.0 / x; }
@@ -224,6 +233,53 @@ __device__ __forceinline__ void boys_taylor32(float T, const float *__restrict__
     }
 }
 
+// F_0..F_L from the order-L slab of the pre-divided table (staged in LDS): Horner in FMAs for
+// F_L, exp(-T) = exp(-T_i) e^{T_i - T} with the 1/k! constants, then downward recursion.
+template <int L, typename Real>
+__device__ __forceinline__ void boys_taylor_lds(Real T, const Real *__restrict__ slab, Real F[L + 1]) {
+    constexpr int TERMS = sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32;
+    constexpr int COLS = TERMS + 1;
+    if (T < Real(BOYS_TMAX)) {
+        int i;
+        if constexpr (sizeof(Real) == 8) i = (int)fmax(T * 10.0 + 0.5, 0.0);
+        else i = (int)fmaxf(T * 10.0f + 0.5f, 0.0f);
+        const Real d = (Real)i * Real(BOYS_GRID_STEP) - T;
+        const Real *row = slab + i * COLS;
+        Real acc = row[TERMS - 1];
+#pragma unroll
+        for (int k = TERMS - 2; k >= 0; --k) acc = acc * d + row[k];
+        F[L] = acc;
+        if constexpr (L > 0) {
+            // e^d = sum_k d^k / k!
+            Real ed = Real(1.0 / 5040.0);
+            if constexpr (TERMS == 8) {
+                ed = ed * d + Real(1.0 / 720.0);
+                ed = ed * d + Real(1.0 / 120.0);
+                ed = ed * d + Real(1.0 / 24.0);
+            } else {
+                ed = Real(1.0 / 24.0);
+            }
+            ed = ed * d + Real(1.0 / 6.0);
+            ed = ed * d + Real(0.5);
+            ed = ed * d + Real(1);
+            ed = ed * d + Real(1);
+            const Real e = row[TERMS] * ed;
+            const Real t2 = Real(2) * T;
+#pragma unroll
+            for (int n = L; n > 0; --n) F[n - 1] = (t2 * F[n] + e) * (Real(1) / Real(2 * n - 1));
+        }
+    } else {
+        static_assert(L <= 4, "extend the grid or keep exp(-T) for higher orders");
+        Real r;
+        if constexpr (sizeof(Real) == 8) r = rsqrt(T);
+        else r = rsqrtf(T);
+        F[0] = Real(0.886226925452758) * r;
+        const Real h = Real(0.5) * r * r;
+#pragma unroll
+        for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * (Real(2 * n - 1) * h);
+    }
+}
+
 // ------------------------------------------------------------------ Hermite
 
 // 1-D Hermite coefficients E[i][j][t] (i <= LA, j <= LB) of one primitive pair along one
@@ -302,7 +358,8 @@ __device__ __forceinline__ void hermite_coulomb(const Real Fs[L + 1], Real X, Re
 template <int LA, int LB, int LC, int LD, int POLICY, typename Real>
 __device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &bra, const PrimPair<Real> &ket,
                                                         const BoysTables &bt,
-                                                        Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)]) {
+                                                        Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)],
+                                                        const Real *lds_slice = nullptr) {
     constexpr int L = LA + LB + LC + LD;
     constexpr int LAB = LA + LB;
     constexpr int LCD = LC + LD;
@@ -327,6 +384,8 @@ __device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &br
         boys_ref<L>((double)T, bt.lut_ref, Fd);
 #pragma unroll
         for (int n = 0; n <= L; ++n) Fs[n] = (Real)Fd[n];
+    } else if (lds_slice) {
+        boys_taylor_lds<L, Real>(T, lds_slice, Fs);
     } else if constexpr (sizeof(Real) == 8) {
         boys_taylor64<L>(T, bt.tab64, Fs);
     } else {

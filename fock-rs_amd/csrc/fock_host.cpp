@@ -106,7 +106,7 @@ struct fock_ctx {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     std::string err;
-    DevBuf lut, tab64, tab32;
+    DevBuf lut, tab64, tab32, pd64, pd32;
     BoysTables bt{};
     Prepared jk;
     bool timing = false;
@@ -142,6 +142,23 @@ long double boys_series(long double T, int m) {
         if (term < 1e-22L * sum) break;
     }
     return expl(-T) * sum;
+}
+
+// Pre-divided slabs: [L][i][k] = F_{L+k}(T_i) / k! (k < terms), [L][i][terms] = exp(-T_i).
+template <typename Real>
+void make_boys_pd(std::vector<Real> &out, int terms) {
+    const int cols = terms + 1;
+    out.assign((size_t)(BOYS_PD_LMAX + 1) * BOYS_GRID_N * cols, Real(0));
+    for (int L = 0; L <= BOYS_PD_LMAX; ++L)
+        for (int i = 0; i < BOYS_GRID_N; ++i) {
+            const long double T = (long double)i / 10.0L;
+            long double fact = 1.0L;
+            for (int k = 0; k < terms; ++k) {
+                if (k > 0) fact *= k;
+                out[((size_t)L * BOYS_GRID_N + i) * cols + k] = (Real)(boys_series(T, L + k) / fact);
+            }
+            out[((size_t)L * BOYS_GRID_N + i) * cols + terms] = (Real)expl(-T);
+        }
 }
 
 void make_boys_tables(std::vector<double> &t64, std::vector<float> &t32) {
@@ -204,6 +221,7 @@ fock_status build_shells(fock_ctx *ctx, const fock_basis_desc *b, std::vector<Sh
         if (f + nf > n) return ctx->fail(FOCK_INVALID_ARG, "incomplete shell at function " + std::to_string(f));
         const int K = b->prim_offset[f + 1] - b->prim_offset[f];
         if (K <= 0) return ctx->fail(FOCK_INVALID_ARG, "function without primitives");
+        if (K > 6) return ctx->fail(FOCK_UNSUPPORTED, "contractions of more than 6 primitives");
         for (int c = 0; c < nf; ++c) {
             const int g = f + c;
             const int *ag = b->angular + 3 * g;
@@ -297,9 +315,19 @@ extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
     HIPCHK(hipMemcpy(ctx->tab64.p, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(ctx->tab32.ensure(t32.size() * sizeof(float)));
     HIPCHK(hipMemcpy(ctx->tab32.p, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<double> p64;
+    std::vector<float> p32;
+    make_boys_pd(p64, BOYS_TAYLOR64);
+    make_boys_pd(p32, BOYS_TAYLOR32);
+    HIPCHK(ctx->pd64.ensure(p64.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->pd64.p, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx->pd32.ensure(p32.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(ctx->pd32.p, p32.data(), p32.size() * sizeof(float), hipMemcpyHostToDevice));
     ctx->bt.lut_ref = ctx->lut.as<double>();
     ctx->bt.tab64 = ctx->tab64.as<double>();
     ctx->bt.tab32 = ctx->tab32.as<float>();
+    ctx->bt.pd64 = ctx->pd64.as<double>();
+    ctx->bt.pd32 = ctx->pd32.as<float>();
     *out = c.release();
     return FOCK_OK;
 }
@@ -747,6 +775,43 @@ CompactBuffers view(Prepared::Compact &c) {
     return CompactBuffers{c.pos.as<int>(),  c.kets.as<int>(), c.head.as<int>(), c.tid.as<int>(),
                           c.kbra.as<int>(), c.kend.as<int>(), c.kcost.as<int>(), c.tasks.as<Task>(),
                           c.count.as<int>(), c.tmp.p, c.tmp_bytes};
+}
+
+// Density-weighted incremental build on the static blocks: J/K of dP over the quartets whose
+// bound Q_ab Q_cd max|dP| reaches tau; bounds >= tau64 run in FP64, the rest in FP32.
+fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, float tau,
+                                           float tau64) {
+    Prepared &pr = ctx->jk;
+    const int n = pr.n, ns = (int)pr.shells.size();
+    HIPCHK(fill_f64((int64_t)n * n, 0.0, Jb, S(ctx)));
+    HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
+    HIPCHK(shell_block_maxabs(ns, pr.d_shell_f0.as<int>(), pr.d_shell_nf.as<int>(), n, dP, pr.d_Pm.as<float>(), S(ctx)));
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        for (int prec = 0; prec < 2; ++prec) {
+            JKArgs a{};
+            a.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
+            a.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
+            a.nbf = n;
+            a.pairs = pr.d_pairs.as<PairInfo>();
+            a.prims = prec == 0 ? pr.d_prims32.p : pr.d_prims64.p;
+            a.P = dP;
+            a.J = Jb;
+            a.K = Kb;
+            a.D = pr.d_D.as<double>();
+            a.dthr = pr.dthr;
+            a.bt = ctx->bt;
+            a.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
+            a.nblocks = pr.block_off[cls + 1] - pr.block_off[cls];
+            a.pair_shell = pr.d_pair_shell.as<int2>();
+            a.pairQ = pr.d_pairQ.as<float>();
+            a.Pm = pr.d_Pm.as<float>();
+            a.nshell = ns;
+            a.bound_lo = prec == 0 ? tau : tau64;
+            a.bound_hi = prec == 0 ? tau64 : HUGE_VALF;
+            HIPCHK(launch_eri_blocks(cls, a, pr.policy, false, prec == 0, S(ctx)));
+        }
+    }
+    return FOCK_OK;
 }
 
 // Density-weighted incremental build: J/K of dP over the quartets whose bound
@@ -1227,15 +1292,20 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 HIPCHK(fill_f64(nn, 0.0, Pprev, S(ctx)));
             }
             HIPCHK(sub_f64(nn, P_, Pprev, dPinc, S(ctx)));
-            STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU, INC_TAU64,
-                                         want_counts));
+            if (n <= BLOCKED_MAX_NBF) {
+                STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU,
+                                                     INC_TAU64));
+            } else {
+                STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU,
+                                             INC_TAU64, want_counts));
+                if (want_counts) evaluated = pr.last_survivors;
+            }
             HIPCHK(hipEventRecord(ev1, S(ctx)));
             HIPCHK(accumulate_g(n, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), Gacc, S(ctx)));
             HIPCHK(copy_f64(nn, P_, Pprev, S(ctx)));
             HIPCHK(copy_f64(nn, Gacc, G_, S(ctx)));
             HIPCHK(add_f64(nn, H_, G_, F_, S(ctx)));
             since_rebuild = (since_rebuild + 1) % REBUILD;
-            if (want_counts) evaluated = pr.last_survivors;
         } else {
             // full build: FP32 quartets in the mixed phase, FP64 otherwise (and in the cleanup)
             STCHK(enqueue_jk(ctx, P_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr, phase == 1));

@@ -33,9 +33,19 @@ struct JKArgs {
     const int *d_ntasks;    // compacted tasks: number of valid tasks on the device (null: ntasks)
     const Block *blocks;    // blocked digestion: groups of tasks sharing one bra pair
     int nblocks;
+    // density-weighted screening (incremental builds): a lane is evaluated only when
+    // Q_ab Q_cd max(2|P_ab|, 2|P_cd|, |P_ac|, |P_ad|, |P_bc|, |P_bd|) lies in [bound_lo, bound_hi)
+    const int2 *pair_shell;  // null: no density screening
+    const float *pairQ;
+    const float *Pm;         // [nshell][nshell] max |P| per shell block
+    int nshell;
+    float bound_lo, bound_hi;
 };
-// largest basis for the blocked digestion: (3 + 3) rows of n doubles in LDS
-constexpr int BLOCKED_MAX_NBF = 1300;
+// largest primitive-pair count of a shell pair (contractions of up to 6 primitives)
+constexpr int MAX_PRIM_PAIRS = 36;
+// largest basis for the blocked digestion: (3 + 3) rows of n doubles in LDS next to the staged
+// Boys slab (at most 33 KB)
+constexpr int BLOCKED_MAX_NBF = 600;
 
 // density-weighted screening of the static task list (compact.hip)
 struct ScreenArgs {
