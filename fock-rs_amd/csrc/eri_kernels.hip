@@ -35,11 +35,27 @@ __device__ __forceinline__ void lane_integrals(const JKArgs &args, const PairInf
 #pragma unroll
         for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
 
-    for (int i = 0; i < bra.K; ++i) {
-        const PrimPair<Real> b = pp[bra.pp_start + i];
-        for (int k = 0; k < ket.K; ++k) {
-            const PrimPair<Real> kq = pp[ket.pp_start + k];
-            accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I);
+    if (args.prim_q) {
+        // primitive pairs are in descending q: once no lane keeps (i, k), none keeps (i, k' > k)
+        for (int i = 0; i < bra.K; ++i) {
+            const PrimPair<Real> b = pp[bra.pp_start + i];
+            const float qb = args.prim_q[bra.pp_start + i];
+            for (int k = 0; k < ket.K; ++k) {
+                const bool keep = qb * args.prim_q[ket.pp_start + k] > args.prim_thr;
+                if (__ballot(keep) == 0) break;
+                if (keep) {
+                    const PrimPair<Real> kq = pp[ket.pp_start + k];
+                    accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I);
+                }
+            }
+        }
+    } else {
+        for (int i = 0; i < bra.K; ++i) {
+            const PrimPair<Real> b = pp[bra.pp_start + i];
+            for (int k = 0; k < ket.K; ++k) {
+                const PrimPair<Real> kq = pp[ket.pp_start + k];
+                accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I);
+            }
         }
     }
 
@@ -73,26 +89,67 @@ __device__ __forceinline__ void lane_integrals(const JKArgs &args, const PairInf
     }
 }
 
-// Same integrals with the Boys slab in LDS and wave-uniform bra primitive pairs: the ket primitive
-// pair is loaded once per lane and reused for every bra primitive.
+// Primitive k of a ket pair from the bucket-SoA copy: base = soa + start, stride = bucket size.
+template <typename Real>
+__device__ __forceinline__ PrimPair<Real> soa_prim(const Real *__restrict__ base, int k, int stride) {
+    const Real *f = base + (size_t)k * PRIM_FIELDS * stride;
+    PrimPair<Real> q;
+    q.p = f[0];
+    q.Px = f[stride];
+    q.Py = f[2 * stride];
+    q.Pz = f[3 * stride];
+    q.Kc = f[4 * stride];
+    q.xax = f[5 * stride];
+    q.xay = f[6 * stride];
+    q.xaz = f[7 * stride];
+    q.xbx = f[8 * stride];
+    q.xby = f[9 * stride];
+    q.xbz = f[10 * stride];
+    q.inv2p = f[11 * stride];
+    return q;
+}
+
+// Same integrals with the bra primitive pairs (and their q) and the Boys slab in LDS, and the
+// ket primitive pairs read from the bucket-SoA copy (the 64 lanes of a task read consecutive
+// entries of every field): each ket primitive pair is loaded once per lane and reused for every
+// bra primitive. ket.pp_start is the pair's SoA start, ket_stride its bucket size.
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
 __device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const PairInfo &bra,
-                                                   const PrimPair<Real> *__restrict__ bra_pp, const PairInfo &ket,
-                                                   bool active,
+                                                   const PrimPair<Real> *__restrict__ bra_pp,
+                                                   const float *__restrict__ bra_q, const PairInfo &ket,
+                                                   int ket_stride, bool active,
                                                    const Real *slice,
                                                    Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)]) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
-    const PrimPair<Real> *__restrict__ pp = reinterpret_cast<const PrimPair<Real> *>(args.prims);
+    const Real *__restrict__ kbase = reinterpret_cast<const Real *>(args.soa) + ket.pp_start;
 #pragma unroll
     for (int x = 0; x < NAB; ++x)
 #pragma unroll
         for (int y = 0; y < NCD; ++y) I[x][y] = Real(0);
-    for (int k = 0; k < ket.K; ++k) {
-        const PrimPair<Real> kq = pp[ket.pp_start + k];
-        for (int i = 0; i < bra.K; ++i) {
-            const PrimPair<Real> b = bra_pp[i];
-            accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+    if (args.prim_q) {
+        // primitive pairs are in descending q (bra and ket): the kept bra primitives of a ket
+        // primitive form a prefix, and the wave stops at the first (k, i) no lane keeps
+        for (int k = 0; k < ket.K; ++k) {
+            const float qk = active ? (float)kbase[((size_t)k * PRIM_FIELDS + 12) * ket_stride] : 0.0f;
+            if (__ballot(qk * bra_q[0] > args.prim_thr) == 0) break;
+            const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
+            for (int i = 0; i < bra.K; ++i) {
+                const bool keep = bra_q[i] * qk > args.prim_thr;
+                if (__ballot(keep) == 0) break;
+                if (keep) {
+                    const PrimPair<Real> b = bra_pp[i];
+                    accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+                }
+            }
+        }
+    } else {
+        for (int k = 0; k < ket.K; ++k) {
+            const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
+            for (int i = 0; i < bra.K; ++i) {
+                const PrimPair<Real> b = bra_pp[i];
+                accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+            }
         }
     }
     const int n = args.nbf;
@@ -284,31 +341,97 @@ __global__ __launch_bounds__(256) void eri_diag_kernel(DiagArgs args) {
         }
 }
 
+// q = sqrt(max_component (ij|ij)) of single primitive pairs (contraction coefficients
+// included), the per-primitive Cauchy-Schwarz factor of the primitive screen.
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void prim_diag_kernel(PrimDiagArgs args) {
+    constexpr int NAB = ncart(LA) * ncart(LB);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= args.count) return;
+    const int idx = args.prim_index[t];
+    const PrimPair<double> pp = args.prims[idx];
+    double I[NAB][NAB];
+#pragma unroll
+    for (int x = 0; x < NAB; ++x)
+#pragma unroll
+        for (int y = 0; y < NAB; ++y) I[x][y] = 0.0;
+    accumulate_prim_quartet<LA, LB, LA, LB, 1, double>(pp, pp, args.bt, I);
+    double m = 0.0;
+#pragma unroll
+    for (int x = 0; x < NAB; ++x) m = fmax(m, fabs(I[x][x]));
+    args.q[idx] = (float)(sqrt(m) * (1.0 + 1e-6));  // rounded up: a bound stays a bound in float
+}
+
+// Number of primitive quartets the primitive screen keeps over a task list.
+__global__ __launch_bounds__(256) void prim_count_kernel(PrimCountArgs args) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    unsigned long long c = 0;
+    if (t < args.ntasks) {
+        const Task task = args.tasks[t];
+        if (lane < task.ket_count) {
+            const PairInfo bra = args.pairs[task.bra], ket = args.pairs[task.ket_begin + lane];
+            for (int k = 0; k < ket.K; ++k) {
+                const float qk = args.prim_q[ket.pp_start + k];
+                for (int i = 0; i < bra.K; ++i) {
+                    if (!(args.prim_q[bra.pp_start + i] * qk > args.prim_thr)) break;
+                    ++c;
+                }
+            }
+        }
+    }
+    c = wave_sum(c);
+    if (lane == 0 && c) atomicAdd(args.out, c);
+}
+
 // Blocked J/K digestion: one workgroup per block of tasks that share a bra pair (a, b). Every K
 // contribution of such a task lands in row a or row b (K[a][c], K[a][d], K[b][c], K[b][d]), so the
 // workgroup accumulates those rows in LDS with LDS atomics and flushes each nonzero entry once;
 // J[a][b] is reduced over the workgroup; only J[c][d] is a per-lane global atomic.
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
-__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimPair<Real> *__restrict__ prims) {
+__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
     constexpr int L = LA + LB + LC + LD;
     constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
-    extern __shared__ double lds[];
+    static_assert(MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) % 16 == 0, "LDS slab alignment");
+    // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | P rows a, b | J_ab partials of
+    // the 4 waves | the block's tasks | their ket pairs
+    extern __shared__ __align__(32) unsigned char lds_bytes[];
     const int n = args.nbf;
+    PrimPair<Real> *bra_pp = reinterpret_cast<PrimPair<Real> *>(lds_bytes);  // [MAX_PRIM_PAIRS]
+    float *bra_q = reinterpret_cast<float *>(bra_pp + MAX_PRIM_PAIRS);       // [MAX_PRIM_PAIRS]
+    Real *slice = reinterpret_cast<Real *>(bra_q + MAX_PRIM_PAIRS);          // [BOYS_GRID_N][COLS]
+    double *lds = reinterpret_cast<double *>(slice + (POLICY == 1 ? BOYS_GRID_N * COLS : 0));
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
-    double *red = Kb + NB * n;        // [4][NAB]
-    Real *slice = reinterpret_cast<Real *>(red + 4 * NAB);  // [BOYS_GRID_N][COLS]
+    double *Pa = Kb + NB * n;         // [NA][n] rows a of P
+    double *Pb = Pa + NA * n;         // [NB][n] rows b of P
+    double *red = Pb + NB * n;        // [4][NAB]
+    Task *s_task = reinterpret_cast<Task *>(red + 4 * NAB);              // [BLOCK_TASKS]
+    PairInfo *s_ket = reinterpret_cast<PairInfo *>(s_task + BLOCK_TASKS);  // [BLOCK_TASKS][64] (SoA starts)
+    int *s_stride = reinterpret_cast<int *>(s_ket + BLOCK_TASKS * 64);     // [BLOCK_TASKS] bucket sizes
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const Block blk = args.blocks[blockIdx.x];
     const PairInfo bra = args.pairs[blk.bra];
-    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
-    // the block's bra primitive pairs are read with wave-uniform addresses from a read-only
-    // (no-alias) argument, which lets them go through the scalar cache
-    const PrimPair<Real> *__restrict__ bra_pp = prims + bra.pp_start;
+    const size_t N = (size_t)n;
+    const double *__restrict__ P = args.P;
+    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) {
+        lds[e] = 0.0;
+        const int row = e / n, col = e - row * n;
+        Pa[e] = P[(size_t)(row < NA ? bra.fa + row : bra.fb + row - NA) * N + col];
+    }
+    if (threadIdx.x < blk.ntasks) {
+        const Task tk = args.tasks[blk.first_task + threadIdx.x];
+        s_task[threadIdx.x] = tk;
+        s_stride[threadIdx.x] = args.soa_stride[tk.ket_begin];
+    }
     {
-        // this class's Boys slab, staged once per workgroup
+        // the block's bra primitive pairs with their q, and this class's Boys slab, staged once
+        const PrimPair<Real> *gpp = reinterpret_cast<const PrimPair<Real> *>(args.prims) + bra.pp_start;
+        for (int e = threadIdx.x; e < bra.K; e += blockDim.x) {
+            bra_pp[e] = gpp[e];
+            bra_q[e] = args.prim_q ? args.prim_q[bra.pp_start + e] : 0.0f;
+        }
         if constexpr (POLICY == 1) {
             const Real *slab;
             if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
@@ -318,21 +441,25 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimP
         }
     }
     __syncthreads();
+    // every ket pair of the block's tasks, loaded in one parallel pass
+    for (int e = threadIdx.x; e < blk.ntasks * 64; e += blockDim.x) {
+        const Task tk = s_task[e >> 6];
+        if ((e & 63) < tk.ket_count) s_ket[e] = args.soa_pairs[tk.ket_begin + (e & 63)];
+    }
+    __syncthreads();
     const int fa = bra.fa, fb = bra.fb;
-    const size_t N = (size_t)n;
-    const double *__restrict__ P = args.P;
     double *__restrict__ J = args.J;
     Real pab[NAB];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)P[(fa + a) * N + (fb + b)];
+        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)Pa[a * n + fb + b];
     double jab[NAB];
 #pragma unroll
     for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
 
     for (int t = wave; t < blk.ntasks; t += 4) {
-        const Task task = args.tasks[blk.first_task + t];
+        const Task task = s_task[t];
         bool active = lane < task.ket_count;
         const int ket_idx = task.ket_begin + (active ? lane : 0);
         if (args.pair_shell) {
@@ -348,9 +475,9 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimP
             active = active && bound >= args.bound_lo && bound < args.bound_hi;
             if (__ballot(active) == 0) continue;  // wave-uniform
         }
-        const PairInfo ket = args.pairs[ket_idx];
+        const PairInfo ket = s_ket[t * 64 + (active ? lane : 0)];
         Real I[NAB][NCD];
-        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, ket, active,
+        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, s_stride[t], active,
                                                                     POLICY == 1 ? slice : nullptr, I);
         const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
@@ -389,7 +516,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimP
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fd + d)];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)Pb[b * n + fd + d];
                 atomicAdd(&Ka[a * n + fc + c], (double)(s * w));
             }
 #pragma unroll
@@ -398,7 +525,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimP
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fc + c)];
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)Pb[b * n + fc + c];
                 atomicAdd(&Ka[a * n + fd + d], (double)(s * w));
             }
         }
@@ -410,7 +537,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimP
 #pragma unroll
                 for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fd + d)];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)Pa[a * n + fd + d];
                 atomicAdd(&Kb[b * n + fc + c], (double)(s * w));
             }
 #pragma unroll
@@ -419,7 +546,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args, const PrimP
 #pragma unroll
                 for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fc + c)];
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)Pa[a * n + fc + c];
                 atomicAdd(&Kb[b * n + fd + d], (double)(s * w));
             }
         }
@@ -476,15 +603,26 @@ static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool ref
                      : launch_one<LA, LB, LC, LD, 1, MODE, false, double>(a, s);
 }
 
+size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32) {
+    const int na = CLASS_BRA[cls] == 0 ? 1 : 3, nb = CLASS_BRA[cls] == 2 ? 3 : 1;
+    const size_t cols = (fp32 ? BOYS_TAYLOR32 : BOYS_TAYLOR64) + 1, real = fp32 ? sizeof(float) : sizeof(double);
+    const size_t pp = fp32 ? sizeof(PrimPair<float>) : sizeof(PrimPair<double>);
+    return 2 * (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
+           BLOCK_TASKS * (sizeof(Task) + 64 * sizeof(PairInfo) + sizeof(int)) + MAX_PRIM_PAIRS * (pp + sizeof(float)) +
+           (policy == 1 ? BOYS_GRID_N * cols * real : 0);
+}
+
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
 static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
     constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
-    const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
+    const size_t lds = 2 * (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
                        4 * ncart(LA) * ncart(LB) * sizeof(double) +
+                       BLOCK_TASKS * (sizeof(Task) + 64 * sizeof(PairInfo) + sizeof(int)) +
+                       MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) +
                        (POLICY == 1 ? (size_t)BOYS_GRID_N * COLS * sizeof(Real) : 0);
     hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real>), dim3(a.nblocks), dim3(256), lds, s,
-                       a, reinterpret_cast<const PrimPair<Real> *>(a.prims));
+                       a);
     return hipGetLastError();
 }
 
@@ -531,6 +669,24 @@ hipError_t launch_eri_tasks(int cls, int mode, const JKArgs &a, int policy, bool
                             hipStream_t s) {
     if (mode == MODE_DENSE) return dispatch_class<MODE_DENSE>(cls, a, policy, refscreen, fp32, s);
     return dispatch_class<MODE_JK>(cls, a, policy, refscreen, fp32, s);
+}
+
+hipError_t launch_prim_diag(int pair_class, const PrimDiagArgs &a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    dim3 grid((a.count + 255) / 256), block(256);
+    switch (pair_class) {
+        case 0: hipLaunchKernelGGL((prim_diag_kernel<0, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((prim_diag_kernel<1, 0>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((prim_diag_kernel<1, 1>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_prim_count(const PrimCountArgs &a, hipStream_t s) {
+    if (a.ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prim_count_kernel, dim3((a.ntasks + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s) {

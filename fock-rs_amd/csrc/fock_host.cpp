@@ -34,10 +34,12 @@ const double REF_LUT_HOST[200] = {
 };
 
 // tasks (64 kets each) per workgroup of the blocked digestion
-constexpr int BLOCK_TASKS = 8;
 
 // f_prim of the canonical McMurchie-Davidson count per quartet class (see fock_jk_stats)
 constexpr double F_PRIM[NUM_CLASSES] = {38, 85, 266, 298, 977, 3833};
+// primitive screen of the accurate policy: a primitive quartet whose Cauchy-Schwarz bound
+// q_ij q_kl (contraction coefficients included) is at most this is not evaluated
+constexpr float PRIM_THRESHOLD = 1e-15f;
 constexpr int CLASS_NINT[NUM_CLASSES] = {1, 3, 9, 9, 27, 81};
 
 struct DevBuf {
@@ -81,6 +83,9 @@ struct Prepared {
     std::vector<Shell> shells;
     std::vector<PairHost> pairs;  // in task (sorted) order
     DevBuf d_pairs, d_prims64, d_prims32, d_D, d_tasks, d_blocks;
+    DevBuf d_prim_q, d_prim_idx, d_count;
+    DevBuf d_soa64, d_soa32, d_soa_pairs, d_soa_stride;  // ket-side primitive data per bucket (SoA)
+    bool prim_screen = false;  // primitive pairs sorted by q and screened (accurate policy)
     int task_off[NUM_CLASSES + 1] = {0};
     int block_off[NUM_CLASSES + 1] = {0};
     fock_jk_stats stats{};
@@ -102,6 +107,7 @@ struct Prepared {
 
 struct fock_ctx {
     int device = 0;
+    size_t max_lds = 65536;  // dynamic LDS a workgroup may allocate on this device
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
@@ -301,6 +307,11 @@ extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
     if (device < 0 || device >= ndev) return FOCK_INVALID_ARG;
     ctx->device = device;
     HIPCHK(hipSetDevice(device));
+    {
+        int lds = 0;
+        HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        ctx->max_lds = (size_t)lds;
+    }
     HIPCHK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
     ctx->stream = ctx->own;
     if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
@@ -443,9 +454,55 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
             pairs.push_back(ph);
         }
     const int npair = (int)pairs.size();
-    HIPCHK(pr.d_prims64.ensure(prims.size() * sizeof(PrimPair<double>)));
-    HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), prims.size() * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
+    const size_t nprim = prims.size();
+    HIPCHK(pr.d_prims64.ensure(nprim * sizeof(PrimPair<double>)));
+    HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), nprim * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
                           S(ctx)));
+    pr.prim_screen = policy == FOCK_POLICY_ACCURATE;
+    std::vector<float> prim_q_host(nprim, 0.0f);
+    if (pr.prim_screen) {
+        // q of every primitive pair, then each shell pair's primitive pairs in descending q
+        // (stable) so that the kept primitive quartets of a ket primitive form a prefix
+        std::vector<int> idx;
+        int start[4] = {0, 0, 0, 0};
+        for (int c = 0; c < 3; ++c) {
+            start[c] = (int)idx.size();
+            for (const PairHost &ph : pairs)
+                if (ph.cls == c)
+                    for (int i = 0; i < ph.K; ++i) idx.push_back(ph.pp_start + i);
+        }
+        start[3] = (int)idx.size();
+        HIPCHK(pr.d_prim_idx.ensure(std::max<size_t>(idx.size(), 1) * sizeof(int)));
+        HIPCHK(pr.d_prim_q.ensure(std::max<size_t>(nprim, 1) * sizeof(float)));
+        HIPCHK(hipMemcpyAsync(pr.d_prim_idx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        for (int c = 0; c < 3; ++c) {
+            PrimDiagArgs da{pr.d_prim_idx.as<int>() + start[c], start[c + 1] - start[c],
+                            pr.d_prims64.as<PrimPair<double>>(), pr.d_prim_q.as<float>(), ctx->bt};
+            HIPCHK(launch_prim_diag(c, da, S(ctx)));
+        }
+        std::vector<float> q(nprim);
+        HIPCHK(hipMemcpyAsync(q.data(), pr.d_prim_q.p, nprim * sizeof(float), hipMemcpyDeviceToHost, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+        std::vector<PrimPair<double>> sorted(nprim);
+        std::vector<float> qs(nprim);
+        std::vector<int> perm;
+        for (const PairHost &ph : pairs) {
+            perm.resize(ph.K);
+            std::iota(perm.begin(), perm.end(), 0);
+            std::stable_sort(perm.begin(), perm.end(),
+                             [&](int x, int y) { return q[ph.pp_start + x] > q[ph.pp_start + y]; });
+            for (int i = 0; i < ph.K; ++i) {
+                sorted[ph.pp_start + i] = prims[ph.pp_start + perm[i]];
+                qs[ph.pp_start + i] = q[ph.pp_start + perm[i]];
+            }
+        }
+        prims.swap(sorted);
+        prim_q_host = qs;
+        HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), nprim * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
+                              S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_prim_q.p, qs.data(), nprim * sizeof(float), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));  // host vectors go out of scope
+    }
     if (precision == 1) {
         std::vector<PrimPair<float>> p32(prims.size());
         for (size_t i = 0; i < prims.size(); ++i) {
@@ -538,6 +595,48 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         while (j < npair && pr.pairs[j].cls == pr.pairs[i].cls && pr.pairs[j].K == pr.pairs[i].K) ++j;
         buckets.push_back(Bucket{pr.pairs[i].cls, pr.pairs[i].K, i, j});
         i = j;
+    }
+    {
+        // ket-side primitive data, structure of arrays per bucket: field f of primitive k of the
+        // bucket's pair j at base + (k * PRIM_FIELDS + f) * nb + (j - begin), so the 64 lanes of a
+        // task (consecutive pairs of one bucket) read each field with one contiguous access
+        std::vector<double> soa64((size_t)nprim * PRIM_FIELDS);
+        std::vector<PairInfo> sinfo(npair);
+        std::vector<int> stride(npair);
+        const std::vector<float> &qh = prim_q_host;
+        const std::vector<PrimPair<double>> &ph = prims;
+        size_t base = 0;
+        for (const Bucket &bk : buckets) {
+            const int nb = bk.end - bk.begin;
+            for (int j = bk.begin; j < bk.end; ++j) {
+                const PairHost &pj = pr.pairs[j];
+                sinfo[j] = PairInfo{(int)(base + (j - bk.begin)), pj.K, pj.fa, pj.fb};
+                stride[j] = nb;
+                for (int k = 0; k < pj.K; ++k) {
+                    const PrimPair<double> &q = ph[pj.pp_start + k];
+                    const double f[PRIM_FIELDS] = {q.p,   q.Px,  q.Py,  q.Pz,  q.Kc,  q.xax,   q.xay,
+                                                   q.xaz, q.xbx, q.xby, q.xbz, q.inv2p, (double)qh[pj.pp_start + k]};
+                    for (int x = 0; x < PRIM_FIELDS; ++x)
+                        soa64[base + ((size_t)k * PRIM_FIELDS + x) * nb + (j - bk.begin)] = f[x];
+                }
+            }
+            base += (size_t)nb * bk.K * PRIM_FIELDS;
+        }
+        HIPCHK(pr.d_soa64.ensure(std::max<size_t>(soa64.size(), 1) * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(pr.d_soa64.p, soa64.data(), soa64.size() * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+        if (precision == 1) {
+            std::vector<float> soa32(soa64.size());
+            for (size_t e = 0; e < soa64.size(); ++e) soa32[e] = (float)soa64[e];
+            HIPCHK(pr.d_soa32.ensure(std::max<size_t>(soa32.size(), 1) * sizeof(float)));
+            HIPCHK(hipMemcpyAsync(pr.d_soa32.p, soa32.data(), soa32.size() * sizeof(float), hipMemcpyHostToDevice,
+                                  S(ctx)));
+            HIPCHK(hipStreamSynchronize(S(ctx)));  // soa32 goes out of scope
+        }
+        HIPCHK(pr.d_soa_pairs.ensure(std::max(npair, 1) * sizeof(PairInfo)));
+        HIPCHK(pr.d_soa_stride.ensure(std::max(npair, 1) * sizeof(int)));
+        HIPCHK(hipMemcpyAsync(pr.d_soa_pairs.p, sinfo.data(), npair * sizeof(PairInfo), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_soa_stride.p, stride.data(), npair * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));  // host arrays go out of scope
     }
     auto keep = [&](double di, double dj) -> bool {
         if (screen == FOCK_SCREEN_NONE) return true;
@@ -680,6 +779,25 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
     HIPCHK(pr.d_shell_nf.ensure(ns * sizeof(int)));
     HIPCHK(hipMemcpyAsync(pr.d_shell_f0.p, f0.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
     HIPCHK(hipMemcpyAsync(pr.d_shell_nf.p, nf.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    if (pr.prim_screen) {
+        // primitive quartets the screen keeps (statistics and the algorithmic flop count)
+        HIPCHK(pr.d_count.ensure(NUM_CLASSES * sizeof(unsigned long long)));
+        HIPCHK(fill_f64(NUM_CLASSES, 0.0, pr.d_count.as<double>(), S(ctx)));  // all-zero bits
+        for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+            PrimCountArgs ca{pr.d_tasks.as<Task>() + pr.task_off[cls], pr.task_off[cls + 1] - pr.task_off[cls],
+                             pr.d_pairs.as<PairInfo>(), pr.d_prim_q.as<float>(), PRIM_THRESHOLD,
+                             pr.d_count.as<unsigned long long>() + cls};
+            HIPCHK(launch_prim_count(ca, S(ctx)));
+        }
+        unsigned long long kept[NUM_CLASSES];
+        HIPCHK(hipMemcpyAsync(kept, pr.d_count.p, sizeof(kept), hipMemcpyDeviceToHost, S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+        for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+            pr.stats.primitive_quartets[cls] = (int64_t)kept[cls];
+            pr.stats.algorithmic_flop[cls] =
+                (double)kept[cls] * F_PRIM[cls] + 12.0 * (double)pr.stats.integrals[cls];
+        }
+    }
     HIPCHK(hipStreamSynchronize(S(ctx)));
     pr.valid = true;
     return FOCK_OK;
@@ -716,6 +834,41 @@ extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
 
 namespace {
 
+// The blocked digestion applies when every class's workgroup fits the device's LDS.
+bool blocked_fits(fock_ctx *ctx, int n, bool fp32) {
+    if (n > BLOCKED_MAX_NBF) return false;
+    for (int c = 0; c < NUM_CLASSES; ++c)
+        if (block_lds_bytes(c, n, ctx->jk.policy, fp32) > ctx->max_lds) return false;
+    return true;
+}
+
+// Kernel arguments shared by every build of class cls over the prepared task lists.
+JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *Jb, double *Kb) {
+    Prepared &pr = ctx->jk;
+    JKArgs a{};
+    a.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
+    a.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
+    a.nbf = pr.n;
+    a.pairs = pr.d_pairs.as<PairInfo>();
+    a.prims = fp32 ? pr.d_prims32.p : pr.d_prims64.p;
+    a.P = dP;
+    a.J = Jb;
+    a.K = Kb;
+    a.D = pr.d_D.as<double>();
+    a.dthr = pr.dthr;
+    a.bt = ctx->bt;
+    a.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
+    a.nblocks = pr.block_off[cls + 1] - pr.block_off[cls];
+    if (pr.prim_screen) {
+        a.prim_q = pr.d_prim_q.as<float>();
+        a.prim_thr = PRIM_THRESHOLD;
+    }
+    a.soa = fp32 ? pr.d_soa32.p : pr.d_soa64.p;
+    a.soa_pairs = pr.d_soa_pairs.as<PairInfo>();
+    a.soa_stride = pr.d_soa_stride.as<int>();
+    return a;
+}
+
 // Enqueue the six class kernels accumulating into Jb/Kb (zeroed here). fp32 selects the FP32
 // quartet kernels (mixed form); default: the prepared precision.
 fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, int mode, double *dense,
@@ -728,23 +881,10 @@ fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, 
     }
     const bool fp32 = (pr.precision == 1) && mode == MODE_JK && !force_fp64;
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
-        JKArgs a{};
-        a.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
-        a.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
-        a.nbf = n;
-        a.pairs = pr.d_pairs.as<PairInfo>();
-        a.prims = fp32 ? pr.d_prims32.p : pr.d_prims64.p;
-        a.P = dP;
-        a.J = Jb;
-        a.K = Kb;
-        a.D = pr.d_D.as<double>();
-        a.dthr = pr.dthr;
-        a.bt = ctx->bt;
+        JKArgs a = class_args(ctx, cls, fp32, dP, Jb, Kb);
         a.dense = dense;
-        a.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
-        a.nblocks = pr.block_off[cls + 1] - pr.block_off[cls];
         if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[cls], S(ctx)));
-        if (mode == MODE_JK && n <= BLOCKED_MAX_NBF)
+        if (mode == MODE_JK && blocked_fits(ctx, n, fp32))
             HIPCHK(launch_eri_blocks(cls, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
         else
             HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
@@ -788,20 +928,7 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
     HIPCHK(shell_block_maxabs(ns, pr.d_shell_f0.as<int>(), pr.d_shell_nf.as<int>(), n, dP, pr.d_Pm.as<float>(), S(ctx)));
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         for (int prec = 0; prec < 2; ++prec) {
-            JKArgs a{};
-            a.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
-            a.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
-            a.nbf = n;
-            a.pairs = pr.d_pairs.as<PairInfo>();
-            a.prims = prec == 0 ? pr.d_prims32.p : pr.d_prims64.p;
-            a.P = dP;
-            a.J = Jb;
-            a.K = Kb;
-            a.D = pr.d_D.as<double>();
-            a.dthr = pr.dthr;
-            a.bt = ctx->bt;
-            a.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
-            a.nblocks = pr.block_off[cls + 1] - pr.block_off[cls];
+            JKArgs a = class_args(ctx, cls, prec == 0, dP, Jb, Kb);
             a.pair_shell = pr.d_pair_shell.as<int2>();
             a.pairQ = pr.d_pairQ.as<float>();
             a.Pm = pr.d_Pm.as<float>();
@@ -836,20 +963,13 @@ fock_status enqueue_jk_incremental(fock_ctx *ctx, const double *dP, double *Jb, 
             const int *flag = prec == 0 ? pr.d_flag32.as<int>() : pr.d_flag64.as<int>();
             HIPCHK(compact_tasks(pr.d_tasks.as<Task>() + off, nt, pr.d_gfirst.as<int>() + off,
                                  pr.d_glast.as<int>() + off, flag, view(c), S(ctx)));
-            JKArgs a{};
+            JKArgs a = class_args(ctx, cls, prec == 0, dP, Jb, Kb);
             a.tasks = c.tasks.as<Task>();
             a.ntasks = nt;
             a.d_ntasks = c.count.as<int>();
             a.ket_list = c.kets.as<int>();
-            a.nbf = n;
-            a.pairs = pr.d_pairs.as<PairInfo>();
-            a.prims = prec == 0 ? pr.d_prims32.p : pr.d_prims64.p;
-            a.P = dP;
-            a.J = Jb;
-            a.K = Kb;
-            a.D = pr.d_D.as<double>();
-            a.dthr = pr.dthr;
-            a.bt = ctx->bt;
+            a.blocks = nullptr;
+            a.nblocks = 0;
             HIPCHK(launch_eri_tasks(cls, MODE_JK, a, pr.policy, false, prec == 0, S(ctx)));
             if (count_survivors) {
                 int last[2];
@@ -1165,6 +1285,15 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     if (!(opts.level_shift >= 0.0)) return ctx->fail(FOCK_INVALID_ARG, "negative level shift");
     const auto t_start = std::chrono::steady_clock::now();
     const bool debug = std::getenv("FOCK_DEBUG") != nullptr;
+    // FOCK_DEBUG: wall time of each phase (synchronises the stream, so only for diagnosis)
+    auto t_mark = t_start;
+    auto phase_mark = [&](const char *what) {
+        if (!debug) return;
+        (void)hipStreamSynchronize(S(ctx));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[fock] %-12s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(now - t_mark).count());
+        t_mark = now;
+    };
     HIPCHK(hipSetDevice(ctx->device));
     const int n = basis->n_functions;
     const int64_t nn = (int64_t)n * n;
@@ -1186,8 +1315,10 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
 
     // direct J/K state (screen lists, pair data)
     const bool mixed = opts.precision == FOCK_PRECISION_MIXED;
+    phase_mark("setup");
     STCHK(fock_jk_prepare(ctx, basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0));
     Prepared &pr = ctx->jk;
+    phase_mark("jk_prepare");
 
     // device matrices
     DevBuf dS, dT, dV, dH, dX, dU, dTmp, dTmp2, dP, dG, dF, dFx, dC, dW, dE, dWork, dInfo, dErrs, dFocks, dDots,
@@ -1217,6 +1348,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         HIPCHK(add_f64(nn, dT.as<double>(), dV.as<double>(), H_, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
     }
+    phase_mark("one_electron");
 
     // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146)
     HIPCHK(copy_f64(nn, S_, T1, S(ctx)));
@@ -1231,7 +1363,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     auto diagonalize = [&](const double *M) -> fock_status {
         HIPCHK(gemm_f64(false, false, n, n, n, 1.0, M, n, X_, n, 0.0, T1, n, S(ctx)));  // M X
         HIPCHK(gemm_f64(true, false, n, n, n, 1.0, X_, n, T1, n, 0.0, T2, n, S(ctx)));  // X^T M X
+        phase_mark("pre_eig");
         STCHK(eig_sym(ctx, n, T2, W_, dWork.as<double>(), dInfo.as<int>()));          // T2 = R'
+        phase_mark("eig");
         HIPCHK(gemm_f64(false, true, n, n, n, 1.0, X_, n, T2, n, 0.0, C_, n, S(ctx)));  // C = X R'^T
         HIPCHK(density_from_coeffs(n, nocc, C_, P_, S(ctx)));
         return FOCK_OK;
@@ -1245,6 +1379,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         STCHK(diagonalize(F_));
     }
 
+    phase_mark("guess");
     // mixed form: phase 0 = FP32 bulk (incremental when requested), phase 1 = FP64 full builds
     int phase = mixed ? 0 : 1;
     const bool incremental = mixed && opts.incremental;
@@ -1292,7 +1427,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 HIPCHK(fill_f64(nn, 0.0, Pprev, S(ctx)));
             }
             HIPCHK(sub_f64(nn, P_, Pprev, dPinc, S(ctx)));
-            if (n <= BLOCKED_MAX_NBF) {
+            if (blocked_fits(ctx, n, true) && blocked_fits(ctx, n, false)) {
                 STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU,
                                                      INC_TAU64));
             } else {
@@ -1312,6 +1447,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             HIPCHK(hipEventRecord(ev1, S(ctx)));
             HIPCHK(fock_from_jk(n, H_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), G_, F_, 0.5, S(ctx)));
         }
+        phase_mark("fock_build");
         // DIIS error e = F P S - S P F = (F P S) - (F P S)^T (F, P, S symmetric) (:189-192)
         const int slot = next_slot;
         next_slot = (next_slot + 1) % 16;
@@ -1367,6 +1503,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             HIPCHK(level_shift_fock(nn, Fuse, S_, T2, opts.level_shift, Fx_, S(ctx)));
             Fuse = Fx_;
         }
+        phase_mark("diis");
         // F' = X^T F X, eigen, C = X C', P_new (:210-216)
         STCHK(diagonalize(Fuse));
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
@@ -1374,6 +1511,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         double e_elec = 0.0;
         HIPCHK(hipMemcpyAsync(&e_elec, scal + 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
+        phase_mark("post_eig");
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         if (out->history && out->history_len < out->history_capacity) {

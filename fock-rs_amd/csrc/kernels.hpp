@@ -40,11 +40,27 @@ struct JKArgs {
     const float *Pm;         // [nshell][nshell] max |P| per shell block
     int nshell;
     float bound_lo, bound_hi;
+    // primitive screening (accurate policy): the primitive quartet (ij|kl) is skipped when
+    // q_ij q_kl <= prim_thr, q = sqrt(max_component (ij|ij)) of the primitive pair including its
+    // contraction coefficients (a Cauchy-Schwarz bound on its contribution). Primitive pairs of
+    // each shell pair are stored in descending q. null: every primitive quartet is evaluated.
+    const float *prim_q;
+    float prim_thr;
+    // ket-side primitive data (blocked kernel): per (class, K) bucket of nb pairs, field f of
+    // primitive k of the bucket's i-th pair at soa[start + (k * PRIM_FIELDS + f) * nb], with
+    // start = soa_pairs[j].pp_start and nb = soa_stride[j]; fields p, Px, Py, Pz, Kc, xa(3),
+    // xb(3), 1/2p, q
+    const void *soa;
+    const PairInfo *soa_pairs;
+    const int *soa_stride;
 };
+constexpr int PRIM_FIELDS = 13;
 // largest primitive-pair count of a shell pair (contractions of up to 6 primitives)
 constexpr int MAX_PRIM_PAIRS = 36;
-// largest basis for the blocked digestion: (3 + 3) rows of n doubles in LDS next to the staged
-// Boys slab (at most 33 KB)
+// tasks of one bra pair per workgroup of the blocked digestion (one wave per task at a time)
+constexpr int BLOCK_TASKS = 8;
+// largest basis for the blocked digestion: 2 x (3 + 3) rows of n doubles (K and P rows) in LDS
+// next to the staged Boys slab (at most 33 KB) and the block's ket pairs (8 KB)
 constexpr int BLOCKED_MAX_NBF = 600;
 
 // density-weighted screening of the static task list (compact.hip)
@@ -83,10 +99,32 @@ struct DiagArgs {
     BoysTables bt;
 };
 
+// q of every primitive pair of one pair class: one thread per primitive pair
+struct PrimDiagArgs {
+    const int *prim_index;  // primitive-pair indices of this class
+    int count;
+    const PrimPair<double> *prims;
+    float *q;  // q[prim_index[t]] = sqrt(max_component (ij|ij))
+    BoysTables bt;
+};
+// surviving primitive quartets of a task list (statistics): out[0] += count
+struct PrimCountArgs {
+    const Task *tasks;
+    int ntasks;
+    const PairInfo *pairs;
+    const float *prim_q;
+    float prim_thr;
+    unsigned long long *out;
+};
+
 hipError_t launch_eri_tasks(int cls, int mode, const JKArgs &a, int policy, bool refscreen, bool fp32,
                             hipStream_t s);
+hipError_t launch_prim_diag(int pair_class, const PrimDiagArgs &a, hipStream_t s);
+hipError_t launch_prim_count(const PrimCountArgs &a, hipStream_t s);
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);
 hipError_t launch_eri_blocks(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s);
+// dynamic LDS of one workgroup of the blocked kernel of class cls
+size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32);
 
 // one_electron.hip
 struct OneElArgs {
