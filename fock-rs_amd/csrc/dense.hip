@@ -301,6 +301,27 @@ hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc,
     return hipGetLastError();
 }
 
+__global__ void scatter_jk_kernel(int npair, const int2 *__restrict__ pair_shell, const int *__restrict__ f0,
+                                  const int *__restrict__ nf, double *__restrict__ Jk, double *__restrict__ J, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npair) return;
+    const int2 ab = pair_shell[j];
+    const int fa = f0[ab.x], fb = f0[ab.y], na = nf[ab.x], nb = nf[ab.y];
+    for (int c = 0; c < na; ++c)
+        for (int d = 0; d < nb; ++d) {
+            double &v = Jk[(size_t)(c * nb + d) * npair + j];
+            J[(size_t)(fa + c) * n + (fb + d)] += v;
+            v = 0.0;
+        }
+}
+hipError_t scatter_jk(int npair, const int2 *pair_shell, const int *shell_f0, const int *shell_nf, double *Jk,
+                      double *J, int n, hipStream_t s) {
+    if (npair <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_jk_kernel, dim3((npair + 255) / 256), dim3(256), 0, s, npair, pair_shell, shell_f0,
+                       shell_nf, Jk, J, n);
+    return hipGetLastError();
+}
+
 __global__ void level_shift_kernel(int64_t len, const double *__restrict__ F, const double *__restrict__ Sm,
                                    const double *__restrict__ SPS, double sigma, double *__restrict__ out) {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

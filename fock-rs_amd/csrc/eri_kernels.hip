@@ -394,8 +394,8 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     constexpr int L = LA + LB + LC + LD;
     constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
     static_assert(MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) % 16 == 0, "LDS slab alignment");
-    // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | P rows a, b | J_ab partials of
-    // the 4 waves | the block's tasks | their ket pairs
+    // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | J_ab partials of the 4 waves |
+    // the block's tasks | its kept ket pairs
     extern __shared__ __align__(32) unsigned char lds_bytes[];
     const int n = args.nbf;
     PrimPair<Real> *bra_pp = reinterpret_cast<PrimPair<Real> *>(lds_bytes);  // [MAX_PRIM_PAIRS]
@@ -404,22 +404,18 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     double *lds = reinterpret_cast<double *>(slice + (POLICY == 1 ? BOYS_GRID_N * COLS : 0));
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
-    double *Pa = Kb + NB * n;         // [NA][n] rows a of P
-    double *Pb = Pa + NA * n;         // [NB][n] rows b of P
-    double *red = Pb + NB * n;        // [4][NAB]
+    double *red = Kb + NB * n;        // [4][NAB]
     Task *s_task = reinterpret_cast<Task *>(red + 4 * NAB);              // [BLOCK_TASKS]
-    PairInfo *s_ket = reinterpret_cast<PairInfo *>(s_task + BLOCK_TASKS);  // [BLOCK_TASKS][64] (SoA starts)
-    int *s_stride = reinterpret_cast<int *>(s_ket + BLOCK_TASKS * 64);     // [BLOCK_TASKS] bucket sizes
+    PairInfo *s_ket = reinterpret_cast<PairInfo *>(s_task + BLOCK_TASKS);  // [BLOCK_TASKS * 64] kept kets (SoA starts)
+    int2 *s_kx = reinterpret_cast<int2 *>(s_ket + BLOCK_TASKS * 64);       // [BLOCK_TASKS * 64] (pair index, bucket size)
+    int *s_stride = reinterpret_cast<int *>(s_kx + BLOCK_TASKS * 64);      // [BLOCK_TASKS] bucket sizes
+    unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_stride + 2 * BLOCK_TASKS);  // [BLOCK_TASKS]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const Block blk = args.blocks[blockIdx.x];
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
     const double *__restrict__ P = args.P;
-    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) {
-        lds[e] = 0.0;
-        const int row = e / n, col = e - row * n;
-        Pa[e] = P[(size_t)(row < NA ? bra.fa + row : bra.fb + row - NA) * N + col];
-    }
+    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
     if (threadIdx.x < blk.ntasks) {
         const Task tk = args.tasks[blk.first_task + threadIdx.x];
         s_task[threadIdx.x] = tk;
@@ -441,10 +437,40 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
         }
     }
     __syncthreads();
-    // every ket pair of the block's tasks, loaded in one parallel pass
-    for (int e = threadIdx.x; e < blk.ntasks * 64; e += blockDim.x) {
-        const Task tk = s_task[e >> 6];
-        if ((e & 63) < tk.ket_count) s_ket[e] = args.soa_pairs[tk.ket_begin + (e & 63)];
+    // the block's kept kets, packed densely in task order: wave w owns task chunks w, w + 4 (a
+    // chunk is one task's 64 ket slots), marks the kets it keeps (all of them, or those passing
+    // the density-weighted screen of an incremental build), then every kept ket is written to
+    // its rank among the kept kets of the block
+    for (int t = wave; t < blk.ntasks; t += 4) {
+        const Task tk = s_task[t];
+        bool keep = lane < tk.ket_count;
+        if (args.pair_shell && keep) {
+            // Q_ab Q_cd max(2|P_ab|, 2|P_cd|, |P_ac|, |P_ad|, |P_bc|, |P_bd|) over shell blocks
+            const int ket_idx = tk.ket_begin + lane;
+            const int2 ab = args.pair_shell[blk.bra], cd = args.pair_shell[ket_idx];
+            const float *Pm = args.Pm;
+            const int ns = args.nshell;
+            float pm = 2.0f * fmaxf(Pm[ab.x * ns + ab.y], Pm[cd.x * ns + cd.y]);
+            pm = fmaxf(pm, fmaxf(Pm[ab.x * ns + cd.x], Pm[ab.x * ns + cd.y]));
+            pm = fmaxf(pm, fmaxf(Pm[ab.y * ns + cd.x], Pm[ab.y * ns + cd.y]));
+            const float bound = args.pairQ[blk.bra] * args.pairQ[ket_idx] * pm;
+            keep = bound >= args.bound_lo && bound < args.bound_hi;
+        }
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) s_mask[t] = m;
+    }
+    __syncthreads();
+    int kept = 0;
+    for (int t = 0; t < blk.ntasks; ++t) kept += __popcll(s_mask[t]);
+    for (int t = wave; t < blk.ntasks; t += 4) {
+        const unsigned long long m = s_mask[t];
+        if ((m >> lane) & 1ull) {
+            int r = __popcll(m & ((1ull << lane) - 1ull));
+            for (int u = 0; u < t; ++u) r += __popcll(s_mask[u]);
+            const Task tk = s_task[t];
+            s_ket[r] = args.soa_pairs[tk.ket_begin + lane];
+            s_kx[r] = make_int2(tk.ket_begin + lane, s_stride[t]);
+        }
     }
     __syncthreads();
     const int fa = bra.fa, fb = bra.fb;
@@ -453,37 +479,26 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)Pa[a * n + fb + b];
+        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)P[(fa + a) * N + (fb + b)];
     double jab[NAB];
 #pragma unroll
     for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
 
-    for (int t = wave; t < blk.ntasks; t += 4) {
-        const Task task = s_task[t];
-        bool active = lane < task.ket_count;
-        const int ket_idx = task.ket_begin + (active ? lane : 0);
-        if (args.pair_shell) {
-            // density-weighted screen of this lane's quartet; tasks without a surviving lane are
-            // skipped by the whole wavefront
-            const int2 ab = args.pair_shell[blk.bra], cd = args.pair_shell[ket_idx];
-            const float *Pm = args.Pm;
-            const int ns = args.nshell;
-            float pm = 2.0f * fmaxf(Pm[ab.x * ns + ab.y], Pm[cd.x * ns + cd.y]);
-            pm = fmaxf(pm, fmaxf(Pm[ab.x * ns + cd.x], Pm[ab.x * ns + cd.y]));
-            pm = fmaxf(pm, fmaxf(Pm[ab.y * ns + cd.x], Pm[ab.y * ns + cd.y]));
-            const float bound = args.pairQ[blk.bra] * args.pairQ[ket_idx] * pm;
-            active = active && bound >= args.bound_lo && bound < args.bound_hi;
-            if (__ballot(active) == 0) continue;  // wave-uniform
-        }
-        const PairInfo ket = s_ket[t * 64 + (active ? lane : 0)];
+    // groups of 64 kept kets, one lane each
+    for (int g = wave * 64; g < kept; g += 256) {
+        const bool active = g + lane < kept;
+        const int slot = active ? g + lane : g;
+        const PairInfo ket = s_ket[slot];
+        const int2 kx = s_kx[slot];
+        const int ket_idx = kx.x;
         Real I[NAB][NCD];
-        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, s_stride[t], active,
+        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, kx.y, active,
                                                                     POLICY == 1 ? slice : nullptr, I);
         const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
         if (fa == fb) w *= Real(0.5);
         if (fc == fd) w *= Real(0.5);
-        if (task.bra == ket_idx) w *= Real(0.5);
+        if (blk.bra == ket_idx) w *= Real(0.5);
         // J[a][b] partial (lane-local; reduced after the loop)
 #pragma unroll
         for (int a = 0; a < NA; ++a)
@@ -497,7 +512,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
                 jab[a * NB + b] += (double)(s * w);
             }
         if (!active) continue;
-        // J[c][d]: global
+        // J[c][d]: pair-indexed accumulator (coalesced over the consecutive ket pairs of a wave)
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -505,7 +520,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
                 Real s = Real(0);
 #pragma unroll
                 for (int x = 0; x < NAB; ++x) s += I[x][c * ND + d] * pab[x];
-                unsafeAtomicAdd(&J[(fc + c) * N + (fd + d)], 2.0 * (double)(s * w));
+                unsafeAtomicAdd(&args.Jk[(size_t)(c * ND + d) * args.npair + ket_idx], 2.0 * (double)(s * w));
             }
         // K rows a and b: LDS
 #pragma unroll
@@ -516,7 +531,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)Pb[b * n + fd + d];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fd + d)];
                 atomicAdd(&Ka[a * n + fc + c], (double)(s * w));
             }
 #pragma unroll
@@ -525,7 +540,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)Pb[b * n + fc + c];
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fc + c)];
                 atomicAdd(&Ka[a * n + fd + d], (double)(s * w));
             }
         }
@@ -537,7 +552,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
 #pragma unroll
                 for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)Pa[a * n + fd + d];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fd + d)];
                 atomicAdd(&Kb[b * n + fc + c], (double)(s * w));
             }
 #pragma unroll
@@ -546,7 +561,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
 #pragma unroll
                 for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)Pa[a * n + fc + c];
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fc + c)];
                 atomicAdd(&Kb[b * n + fd + d], (double)(s * w));
             }
         }
@@ -607,8 +622,10 @@ size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32) {
     const int na = CLASS_BRA[cls] == 0 ? 1 : 3, nb = CLASS_BRA[cls] == 2 ? 3 : 1;
     const size_t cols = (fp32 ? BOYS_TAYLOR32 : BOYS_TAYLOR64) + 1, real = fp32 ? sizeof(float) : sizeof(double);
     const size_t pp = fp32 ? sizeof(PrimPair<float>) : sizeof(PrimPair<double>);
-    return 2 * (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
-           BLOCK_TASKS * (sizeof(Task) + 64 * sizeof(PairInfo) + sizeof(int)) + MAX_PRIM_PAIRS * (pp + sizeof(float)) +
+    return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
+           BLOCK_TASKS * (sizeof(Task) + 64 * (sizeof(PairInfo) + sizeof(int2)) + 2 * sizeof(int) +
+                          sizeof(unsigned long long)) +
+           MAX_PRIM_PAIRS * (pp + sizeof(float)) +
            (policy == 1 ? BOYS_GRID_N * cols * real : 0);
 }
 
@@ -616,9 +633,10 @@ template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename R
 static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
     constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
-    const size_t lds = 2 * (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
+    const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
                        4 * ncart(LA) * ncart(LB) * sizeof(double) +
-                       BLOCK_TASKS * (sizeof(Task) + 64 * sizeof(PairInfo) + sizeof(int)) +
+                       BLOCK_TASKS * (sizeof(Task) + 64 * (sizeof(PairInfo) + sizeof(int2)) + 2 * sizeof(int) +
+                                      sizeof(unsigned long long)) +
                        MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) +
                        (POLICY == 1 ? (size_t)BOYS_GRID_N * COLS * sizeof(Real) : 0);
     hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real>), dim3(a.nblocks), dim3(256), lds, s,

@@ -85,6 +85,7 @@ struct Prepared {
     DevBuf d_pairs, d_prims64, d_prims32, d_D, d_tasks, d_blocks;
     DevBuf d_prim_q, d_prim_idx, d_count;
     DevBuf d_soa64, d_soa32, d_soa_pairs, d_soa_stride;  // ket-side primitive data per bucket (SoA)
+    DevBuf d_Jk;  // ket-side J of the blocked kernel, [9][npair], all-zero between builds
     bool prim_screen = false;  // primitive pairs sorted by q and screened (accurate policy)
     int task_off[NUM_CLASSES + 1] = {0};
     int block_off[NUM_CLASSES + 1] = {0};
@@ -747,23 +748,29 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
     }
     pr.stats = st;
     pr.compact_ready = false;
-    if (precision == 1) {
+    {
         std::vector<int2> psh(npair);
         std::vector<float> pq(npair);
         for (int i = 0; i < npair; ++i) {
             psh[i] = make_int2(pr.pairs[i].sa, pr.pairs[i].sb);
             pq[i] = (float)std::sqrt(std::max(pr.pairs[i].dmax, 0.0));
         }
+        HIPCHK(pr.d_pair_shell.ensure(npair * sizeof(int2)));
+        HIPCHK(pr.d_pairQ.ensure(npair * sizeof(float)));
+        HIPCHK(hipMemcpyAsync(pr.d_pair_shell.p, psh.data(), npair * sizeof(int2), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_pairQ.p, pq.data(), npair * sizeof(float), hipMemcpyHostToDevice, S(ctx)));
+        // ket-side J accumulator of the blocked kernel (kept all-zero between builds)
+        HIPCHK(pr.d_Jk.ensure((size_t)9 * npair * sizeof(double)));
+        HIPCHK(fill_f64((int64_t)9 * npair, 0.0, pr.d_Jk.as<double>(), S(ctx)));
+        HIPCHK(hipStreamSynchronize(S(ctx)));
+    }
+    if (precision == 1) {
         const size_t nt = std::max<size_t>(tasks.size(), 1);
         HIPCHK(pr.d_gfirst.ensure(nt * sizeof(int)));
         HIPCHK(pr.d_glast.ensure(nt * sizeof(int)));
-        HIPCHK(pr.d_pair_shell.ensure(npair * sizeof(int2)));
-        HIPCHK(pr.d_pairQ.ensure(npair * sizeof(float)));
         HIPCHK(pr.d_Pm.ensure((size_t)ns * ns * sizeof(float)));
         HIPCHK(hipMemcpyAsync(pr.d_gfirst.p, gfirst_all.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
         HIPCHK(hipMemcpyAsync(pr.d_glast.p, glast_all.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
-        HIPCHK(hipMemcpyAsync(pr.d_pair_shell.p, psh.data(), npair * sizeof(int2), hipMemcpyHostToDevice, S(ctx)));
-        HIPCHK(hipMemcpyAsync(pr.d_pairQ.p, pq.data(), npair * sizeof(float), hipMemcpyHostToDevice, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
     }
     HIPCHK(pr.d_tasks.ensure(std::max<size_t>(tasks.size(), 1) * sizeof(Task)));
@@ -834,6 +841,13 @@ extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
 
 namespace {
 
+// Ket-side J of the blocked kernels (pair-indexed) into Jb; leaves the accumulator zeroed.
+hipError_t scatter_pair_j(fock_ctx *ctx, double *Jb) {
+    Prepared &pr = ctx->jk;
+    return scatter_jk((int)pr.pairs.size(), pr.d_pair_shell.as<int2>(), pr.d_shell_f0.as<int>(),
+                      pr.d_shell_nf.as<int>(), pr.d_Jk.as<double>(), Jb, pr.n, S(ctx));
+}
+
 // The blocked digestion applies when every class's workgroup fits the device's LDS.
 bool blocked_fits(fock_ctx *ctx, int n, bool fp32) {
     if (n > BLOCKED_MAX_NBF) return false;
@@ -866,6 +880,8 @@ JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *J
     a.soa = fp32 ? pr.d_soa32.p : pr.d_soa64.p;
     a.soa_pairs = pr.d_soa_pairs.as<PairInfo>();
     a.soa_stride = pr.d_soa_stride.as<int>();
+    a.Jk = pr.d_Jk.as<double>();
+    a.npair = (int)pr.pairs.size();
     return a;
 }
 
@@ -880,15 +896,17 @@ fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, 
         HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
     }
     const bool fp32 = (pr.precision == 1) && mode == MODE_JK && !force_fp64;
+    const bool blocked = mode == MODE_JK && blocked_fits(ctx, n, fp32);
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         JKArgs a = class_args(ctx, cls, fp32, dP, Jb, Kb);
         a.dense = dense;
         if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[cls], S(ctx)));
-        if (mode == MODE_JK && blocked_fits(ctx, n, fp32))
+        if (blocked)
             HIPCHK(launch_eri_blocks(cls, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
         else
             HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
     }
+    if (blocked) HIPCHK(scatter_pair_j(ctx, Jb));
     if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[NUM_CLASSES], S(ctx)));
     return FOCK_OK;
 }
@@ -928,6 +946,7 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
     HIPCHK(shell_block_maxabs(ns, pr.d_shell_f0.as<int>(), pr.d_shell_nf.as<int>(), n, dP, pr.d_Pm.as<float>(), S(ctx)));
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         for (int prec = 0; prec < 2; ++prec) {
+            if (prec == 1 && !(tau64 < HUGE_VALF)) continue;  // FP32 only
             JKArgs a = class_args(ctx, cls, prec == 0, dP, Jb, Kb);
             a.pair_shell = pr.d_pair_shell.as<int2>();
             a.pairQ = pr.d_pairQ.as<float>();
@@ -938,6 +957,7 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
             HIPCHK(launch_eri_blocks(cls, a, pr.policy, false, prec == 0, S(ctx)));
         }
     }
+    HIPCHK(scatter_pair_j(ctx, Jb));
     return FOCK_OK;
 }
 
@@ -1380,13 +1400,22 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     }
 
     phase_mark("guess");
-    // mixed form: phase 0 = FP32 bulk (incremental when requested), phase 1 = FP64 full builds
+    // mixed form. Phase 0 (bulk): FP32 quartets; with incremental builds G(P_k) = G(P_{k-1}) +
+    // G(P_k - P_{k-1}) over the density-weighted screen, all in FP32. Phase 1: without incremental
+    // builds, FP64 full builds; with them, an FP64 full build that resets the accumulated G, then
+    // incremental builds with dynamic precision (bounds >= INC_TAU64 in FP64), an FP64 full build
+    // every REBUILD of them, and an FP64 full cleanup build whenever an incremental build reaches
+    // epsilon: the result always comes from an FP64 full build.
     int phase = mixed ? 0 : 1;
     const bool incremental = mixed && opts.incremental;
-    constexpr int REBUILD = 8;
-    constexpr float INC_TAU = 1e-11f, INC_TAU64 = 1e-5f;
-    constexpr double MIXED_SWITCH = 1e-5;
+    constexpr int REBUILD = 12;
+    // drop thresholds of the density-weighted bound: looser in the bulk phase (whose errors the
+    // FP64 rebuild discards), tight in phase 1; bounds >= INC_TAU64 run in FP64 there (the FP32
+    // error of such a quartet, ~1e-7 of its bound, stays below 1e-11)
+    constexpr float INC_TAU_BULK = 1e-9f, INC_TAU = 1e-11f, INC_TAU64 = 1e-4f;
+    constexpr double MIXED_SWITCH = 1e-4;
     int since_rebuild = 0, stall = 0;
+    bool need_full = false;  // next build of phase 1 is an FP64 full build
     double best_err = HUGE_VAL;
     const bool want_counts = out->history != nullptr;
     int64_t static_quartets = 0;
@@ -1420,19 +1449,20 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         // G(P), F = H + G (scf/src/lib.rs:177-187)
         HIPCHK(hipEventRecord(ev0, S(ctx)));
         int64_t evaluated = static_quartets;
-        if (phase == 0 && incremental) {
-            // G(P_k) = G(P_{k-1}) + G(P_k - P_{k-1}); full rebuild every REBUILD iterations
-            if (since_rebuild == 0) {
+        const bool incr_build = incremental && (phase == 0 || !need_full);
+        if (incr_build) {
+            if (iter == 0) {  // G(0) = 0
                 HIPCHK(fill_f64(nn, 0.0, Gacc, S(ctx)));
                 HIPCHK(fill_f64(nn, 0.0, Pprev, S(ctx)));
             }
             HIPCHK(sub_f64(nn, P_, Pprev, dPinc, S(ctx)));
+            const float tau = phase == 0 ? INC_TAU_BULK : INC_TAU, tau64 = phase == 0 ? HUGE_VALF : INC_TAU64;
             if (blocked_fits(ctx, n, true) && blocked_fits(ctx, n, false)) {
-                STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU,
-                                                     INC_TAU64));
+                STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau,
+                                                     tau64));
             } else {
-                STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), INC_TAU,
-                                             INC_TAU64, want_counts));
+                STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau, tau64,
+                                             want_counts));
                 if (want_counts) evaluated = pr.last_survivors;
             }
             HIPCHK(hipEventRecord(ev1, S(ctx)));
@@ -1440,12 +1470,18 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             HIPCHK(copy_f64(nn, P_, Pprev, S(ctx)));
             HIPCHK(copy_f64(nn, Gacc, G_, S(ctx)));
             HIPCHK(add_f64(nn, H_, G_, F_, S(ctx)));
-            since_rebuild = (since_rebuild + 1) % REBUILD;
+            if (phase == 1 && ++since_rebuild >= REBUILD) need_full = true;
         } else {
-            // full build: FP32 quartets in the mixed phase, FP64 otherwise (and in the cleanup)
+            // full build: FP32 quartets in the bulk phase, FP64 otherwise (and in every cleanup)
             STCHK(enqueue_jk(ctx, P_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr, phase == 1));
             HIPCHK(hipEventRecord(ev1, S(ctx)));
             HIPCHK(fock_from_jk(n, H_, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), G_, F_, 0.5, S(ctx)));
+            if (incremental) {  // restart the increments from this exact G
+                HIPCHK(copy_f64(nn, G_, Gacc, S(ctx)));
+                HIPCHK(copy_f64(nn, P_, Pprev, S(ctx)));
+                need_full = false;
+                since_rebuild = 0;
+            }
         }
         phase_mark("fock_build");
         // DIIS error e = F P S - S P F = (F P S) - (F P S)^T (F, P, S symmetric) (:189-192)
@@ -1537,7 +1573,19 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 ++stall;
             }
             const bool near = err < std::max(10.0 * opts.epsilon, MIXED_SWITCH);
-            if (near || stall >= 3 || iter + 3 >= opts.max_iters) phase = 1;
+            if (near || stall >= 3 || iter + 3 >= opts.max_iters) {
+                phase = 1;
+                need_full = true;
+            }
+            if (!is_normal(err) && !(err < opts.epsilon)) {
+                result = FOCK_DIVERGED;
+                break;
+            }
+            continue;
+        }
+        if (incr_build && iter < opts.max_iters) {
+            // an incremental result is never final: reaching epsilon asks for the FP64 cleanup
+            if (err < opts.epsilon || iter + 1 == opts.max_iters) need_full = true;
             if (!is_normal(err) && !(err < opts.epsilon)) {
                 result = FOCK_DIVERGED;
                 break;

@@ -53,14 +53,19 @@ struct JKArgs {
     const void *soa;
     const PairInfo *soa_pairs;
     const int *soa_stride;
+    // ket-side J of the blocked kernel, by pair and component: J[fc + c][fd + d] of ket pair j
+    // accumulates in Jk[(c * ND + d) * npair + j] (consecutive lanes hold consecutive pairs, so
+    // every atomic instruction covers a few cache lines), scattered into J after the build
+    double *Jk;
+    int npair;
 };
 constexpr int PRIM_FIELDS = 13;
 // largest primitive-pair count of a shell pair (contractions of up to 6 primitives)
 constexpr int MAX_PRIM_PAIRS = 36;
 // tasks of one bra pair per workgroup of the blocked digestion (one wave per task at a time)
 constexpr int BLOCK_TASKS = 8;
-// largest basis for the blocked digestion: 2 x (3 + 3) rows of n doubles (K and P rows) in LDS
-// next to the staged Boys slab (at most 33 KB) and the block's ket pairs (8 KB)
+// largest basis for the blocked digestion: (3 + 3) K rows of n doubles in LDS next to the staged
+// Boys slab (at most 33 KB) and the block's ket records (13 KB)
 constexpr int BLOCKED_MAX_NBF = 600;
 
 // density-weighted screening of the static task list (compact.hip)
@@ -87,6 +92,9 @@ hipError_t compact_tasks(const Task *tasks, int ntasks, const int *gfirst, const
                          const CompactBuffers &b, hipStream_t s);
 // Gacc += (Jb + Jb^T) - 0.5 (Kb + Kb^T)
 hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc, hipStream_t s);
+// J[f0(a) + c][f0(b) + d] += Jk[(c * nf(b) + d) * npair + j] for every pair j = (a, b), then Jk = 0
+hipError_t scatter_jk(int npair, const int2 *pair_shell, const int *shell_f0, const int *shell_nf, double *Jk,
+                      double *J, int n, hipStream_t s);
 // out = F + sigma (S - 0.5 SPS)
 hipError_t level_shift_fock(int64_t len, const double *F, const double *S, const double *SPS, double sigma,
                             double *out, hipStream_t s);
