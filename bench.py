@@ -141,7 +141,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip both SCF runs (builds use a synthetic density)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Fock build is measured on a single MI355X "
@@ -149,6 +151,17 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+
+    if args.calibrate_copy:
+        # the byte count a plain copy kernel moves, for scaling FETCH_SIZE / WRITE_SIZE
+        n_bytes = args.calibrate_copy << 20
+        x = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n_bytes, "launches": 3}), flush=True)
+        return
 
     from fock_amd import abi, molecule as m
 
@@ -170,7 +183,7 @@ def main():
     scf = {}
     P_ref = None
     for name, prec, incr in (("fp64", abi.PRECISION_FP64, 0), ("mixed", abi.PRECISION_MIXED, 1)):
-        if args.no_scf and name == "mixed":
+        if args.no_scf:
             continue
         opts = abi.default_opts(max_iters=100, epsilon=SCF_EPS, integral_policy=abi.POLICY_ACCURATE,
                                 screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12, precision=prec,
@@ -192,7 +205,7 @@ def main():
         if name == "fp64":
             P_ref = r["density"]
         log("SCF %s: %s" % (name, scf[name]))
-    if P_ref is None:
+    if P_ref is None:  # synthetic density (same screened quartet set: the build does not screen on P)
         P_ref = np.eye(n) * 0.1
 
     dP = torch.from_numpy(np.ascontiguousarray(P_ref)).to("cuda:0")

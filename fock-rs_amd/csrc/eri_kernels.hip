@@ -415,32 +415,15 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
     const double *__restrict__ P = args.P;
-    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
     if (threadIdx.x < blk.ntasks) {
         const Task tk = args.tasks[blk.first_task + threadIdx.x];
         s_task[threadIdx.x] = tk;
         s_stride[threadIdx.x] = args.soa_stride[tk.ket_begin];
     }
-    {
-        // the block's bra primitive pairs with their q, and this class's Boys slab, staged once
-        const PrimPair<Real> *gpp = reinterpret_cast<const PrimPair<Real> *>(args.prims) + bra.pp_start;
-        for (int e = threadIdx.x; e < bra.K; e += blockDim.x) {
-            bra_pp[e] = gpp[e];
-            bra_q[e] = args.prim_q ? args.prim_q[bra.pp_start + e] : 0.0f;
-        }
-        if constexpr (POLICY == 1) {
-            const Real *slab;
-            if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
-            else slab = reinterpret_cast<const Real *>(args.bt.pd32);
-            slab += (size_t)L * BOYS_GRID_N * COLS;
-            for (int e = threadIdx.x; e < BOYS_GRID_N * COLS; e += blockDim.x) slice[e] = slab[e];
-        }
-    }
     __syncthreads();
     // the block's kept kets, packed densely in task order: wave w owns task chunks w, w + 4 (a
-    // chunk is one task's 64 ket slots), marks the kets it keeps (all of them, or those passing
-    // the density-weighted screen of an incremental build), then every kept ket is written to
-    // its rank among the kept kets of the block
+    // chunk is one task's 64 ket slots) and marks the kets it keeps (all of them, or those passing
+    // the density-weighted screen of an incremental build)
     for (int t = wave; t < blk.ntasks; t += 4) {
         const Task tk = s_task[t];
         bool keep = lane < tk.ket_count;
@@ -462,6 +445,9 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     __syncthreads();
     int kept = 0;
     for (int t = 0; t < blk.ntasks; ++t) kept += __popcll(s_mask[t]);
+    if (kept == 0) return;  // the same for every thread of the block
+    // every kept ket is written to its rank among the block's kept kets; the K rows, the bra
+    // primitive pairs with their q and this class's Boys slab are staged
     for (int t = wave; t < blk.ntasks; t += 4) {
         const unsigned long long m = s_mask[t];
         if ((m >> lane) & 1ull) {
@@ -470,6 +456,21 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
             const Task tk = s_task[t];
             s_ket[r] = args.soa_pairs[tk.ket_begin + lane];
             s_kx[r] = make_int2(tk.ket_begin + lane, s_stride[t]);
+        }
+    }
+    for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
+    {
+        const PrimPair<Real> *gpp = reinterpret_cast<const PrimPair<Real> *>(args.prims) + bra.pp_start;
+        for (int e = threadIdx.x; e < bra.K; e += blockDim.x) {
+            bra_pp[e] = gpp[e];
+            bra_q[e] = args.prim_q ? args.prim_q[bra.pp_start + e] : 0.0f;
+        }
+        if constexpr (POLICY == 1) {
+            const Real *slab;
+            if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
+            else slab = reinterpret_cast<const Real *>(args.bt.pd32);
+            slab += (size_t)L * BOYS_GRID_N * COLS;
+            for (int e = threadIdx.x; e < BOYS_GRID_N * COLS; e += blockDim.x) slice[e] = slab[e];
         }
     }
     __syncthreads();

@@ -87,6 +87,7 @@ struct Prepared {
     DevBuf d_soa64, d_soa32, d_soa_pairs, d_soa_stride;  // ket-side primitive data per bucket (SoA)
     DevBuf d_Jk;  // ket-side J of the blocked kernel, [9][npair], all-zero between builds
     bool prim_screen = false;  // primitive pairs sorted by q and screened (accurate policy)
+    bool prim_counted = false;  // stats.primitive_quartets holds the kept (screened) count
     int task_off[NUM_CLASSES + 1] = {0};
     int block_off[NUM_CLASSES + 1] = {0};
     fock_jk_stats stats{};
@@ -786,8 +787,20 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
     HIPCHK(pr.d_shell_nf.ensure(ns * sizeof(int)));
     HIPCHK(hipMemcpyAsync(pr.d_shell_f0.p, f0.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
     HIPCHK(hipMemcpyAsync(pr.d_shell_nf.p, nf.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
-    if (pr.prim_screen) {
-        // primitive quartets the screen keeps (statistics and the algorithmic flop count)
+    pr.prim_counted = !pr.prim_screen;  // kept primitive quartets: counted on the first stats request
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    pr.valid = true;
+    return FOCK_OK;
+}
+
+namespace {
+
+// Primitive quartets the primitive screen keeps, per class (statistics and the algorithmic flop
+// count of the accurate policy); one pass over the task lists.
+fock_status count_kept_primitives(fock_ctx *ctx) {
+    Prepared &pr = ctx->jk;
+    if (pr.prim_counted) return FOCK_OK;
+    {
         HIPCHK(pr.d_count.ensure(NUM_CLASSES * sizeof(unsigned long long)));
         HIPCHK(fill_f64(NUM_CLASSES, 0.0, pr.d_count.as<double>(), S(ctx)));  // all-zero bits
         for (int cls = 0; cls < NUM_CLASSES; ++cls) {
@@ -805,10 +818,11 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
                 (double)kept[cls] * F_PRIM[cls] + 12.0 * (double)pr.stats.integrals[cls];
         }
     }
-    HIPCHK(hipStreamSynchronize(S(ctx)));
-    pr.valid = true;
+    pr.prim_counted = true;
     return FOCK_OK;
 }
+
+}  // namespace
 
 extern "C" fock_status fock_jk_set_timing(fock_ctx *ctx, int32_t enable) {
     if (!ctx) return FOCK_INVALID_ARG;
@@ -835,6 +849,8 @@ extern "C" fock_status fock_jk_get_timings(fock_ctx *ctx, double ms[6]) {
 extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
     if (!ctx || !stats) return FOCK_INVALID_ARG;
     if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
+    HIPCHK(hipSetDevice(ctx->device));
+    STCHK(count_kept_primitives(ctx));
     *stats = ctx->jk.stats;
     return FOCK_OK;
 }
