@@ -187,14 +187,21 @@ __device__ __forceinline__ void boys_taylor64(double T, const double *__restrict
             for (int n = L; n > 0; --n) F[n - 1] = fma(t2, F[n], e) * (1.0 / (2 * n - 1));
         }
     } else {
-        // upward recursion without the exp(-T) term: relative error below 1e-14 for the
-        // orders of s/p quartets (L <= 4) at T >= 45
-        static_assert(L <= 4, "extend the grid or keep exp(-T) for higher orders");
         double r = rsqrt(T);
-        F[0] = 0.886226925452758 * r;  // sqrt(pi)/2 / sqrt(T)
+        F[0] = 0.886226925452758 * r;  // sqrt(pi)/2 / sqrt(T) (erf(sqrt T) = 1 to 1e-21)
         double h = 0.5 * r * r;        // 1/(2T)
+        if constexpr (L <= 4) {
+            // upward recursion without the exp(-T) term: relative error below 1e-14 for the
+            // orders of s/p quartets (L <= 4) at T >= 45
 #pragma unroll
-        for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((2 * n - 1) * h);
+            for (int n = 1; n <= L; ++n) F[n] = F[n - 1] * ((2 * n - 1) * h);
+        } else {
+            // higher orders (derivative integrals): the exact upward recursion
+            // F_n = ((2n - 1) F_{n-1} - exp(-T)) / (2T)
+            const double e = exp(-T);
+#pragma unroll
+            for (int n = 1; n <= L; ++n) F[n] = ((2 * n - 1) * F[n - 1] - e) * h;
+        }
     }
 }
 

@@ -86,6 +86,7 @@ struct Prepared {
     DevBuf d_prim_q, d_prim_idx, d_count;
     DevBuf d_soa64, d_soa32, d_soa_pairs, d_soa_stride;  // ket-side primitive data per bucket (SoA)
     DevBuf d_Jk;  // ket-side J of the blocked kernel, [9][npair], all-zero between builds
+    DevBuf d_prim_ab;  // exponents of both primitives of every primitive pair (gradients)
     bool prim_screen = false;  // primitive pairs sorted by q and screened (accurate policy)
     bool prim_counted = false;  // stats.primitive_quartets holds the kept (screened) count
     int task_off[NUM_CLASSES + 1] = {0};
@@ -437,6 +438,7 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
     // every shell pair, oriented L(a) >= L(b) (same L: higher shell index first)
     std::vector<PairHost> pairs;
     std::vector<PrimPair<double>> prims;
+    std::vector<double2> prim_ab;  // exponents of the two primitives of every primitive pair
     for (int s1 = 0; s1 < ns; ++s1)
         for (int s2 = 0; s2 <= s1; ++s2) {
             int a = s1, b = s2;
@@ -452,7 +454,11 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
             ph.creation = (int)pairs.size();
             ph.dmax = 0.0;
             for (int i = 0; i < pr.shells[a].K; ++i)
-                for (int j = 0; j < pr.shells[b].K; ++j) prims.push_back(make_prim_pair(basis, pr.shells[a], pr.shells[b], i, j));
+                for (int j = 0; j < pr.shells[b].K; ++j) {
+                    prims.push_back(make_prim_pair(basis, pr.shells[a], pr.shells[b], i, j));
+                    prim_ab.push_back(make_double2(basis->exponents[pr.shells[a].prim0 + i],
+                                                   basis->exponents[pr.shells[b].prim0 + j]));
+                }
             pairs.push_back(ph);
         }
     const int npair = (int)pairs.size();
@@ -486,6 +492,7 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         HIPCHK(hipMemcpyAsync(q.data(), pr.d_prim_q.p, nprim * sizeof(float), hipMemcpyDeviceToHost, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
         std::vector<PrimPair<double>> sorted(nprim);
+        std::vector<double2> sorted_ab(nprim);
         std::vector<float> qs(nprim);
         std::vector<int> perm;
         for (const PairHost &ph : pairs) {
@@ -495,10 +502,12 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
                              [&](int x, int y) { return q[ph.pp_start + x] > q[ph.pp_start + y]; });
             for (int i = 0; i < ph.K; ++i) {
                 sorted[ph.pp_start + i] = prims[ph.pp_start + perm[i]];
+                sorted_ab[ph.pp_start + i] = prim_ab[ph.pp_start + perm[i]];
                 qs[ph.pp_start + i] = q[ph.pp_start + perm[i]];
             }
         }
         prims.swap(sorted);
+        prim_ab.swap(sorted_ab);
         prim_q_host = qs;
         HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), nprim * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
                               S(ctx)));
@@ -518,6 +527,10 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
                               S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
     }
+
+    HIPCHK(pr.d_prim_ab.ensure(std::max<size_t>(nprim, 1) * sizeof(double2)));
+    HIPCHK(hipMemcpyAsync(pr.d_prim_ab.p, prim_ab.data(), nprim * sizeof(double2), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));  // prim_ab is a host temporary
 
     // diagonal (ab|ab): pairs grouped by class for the class-templated kernel
     std::vector<int> by_class(npair);
@@ -1301,6 +1314,107 @@ extern "C" fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const do
     HIPCHK(lincomb(len, dF.as<double>(), len, dSl.as<int>(), dC.as<double>(), m, dO.as<double>(), S(ctx)));
     HIPCHK(hipMemcpyAsync(out, dO.p, len * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
     HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
+                                         const fock_scf_opts *opts_in, const fock_scf_out *scf, double *gradient) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (!basis || !nuclei || !scf || !gradient) return ctx->fail(FOCK_INVALID_ARG, "null argument");
+    fock_scf_opts opts;
+    if (opts_in) opts = *opts_in;
+    else fock_scf_opts_default(&opts);
+    if (opts.integral_policy != FOCK_POLICY_ACCURATE)
+        return ctx->fail(FOCK_INVALID_ARG, "gradients are defined for the accurate integral policy only");
+    if (!scf->density || !scf->coefficients || !scf->orbital_energies)
+        return ctx->fail(FOCK_INVALID_ARG, "the SCF result must hold density, coefficients and orbital energies");
+    const int n = basis->n_functions, na = nuclei->n_atoms;
+    if (scf->n_basis != n || na <= 0) return ctx->fail(FOCK_INVALID_ARG, "SCF result does not match the basis");
+    const int nocc = scf->n_electrons / 2;
+    if (nocc < 0 || nocc > n) return ctx->fail(FOCK_INVALID_ARG, "bad electron count");
+    if ((size_t)3 * na * sizeof(double) > ctx->max_lds) return ctx->fail(FOCK_UNSUPPORTED, "too many atoms");
+    HIPCHK(hipSetDevice(ctx->device));
+    STCHK(fock_jk_prepare(ctx, basis, FOCK_POLICY_ACCURATE, opts.screen, opts.schwarz_threshold, 0));
+    Prepared &pr = ctx->jk;
+    const int ns = (int)pr.shells.size();
+    // the nucleus each shell sits on (derivatives move basis functions with their nucleus)
+    std::vector<int> atom(ns);
+    for (int s = 0; s < ns; ++s) {
+        atom[s] = -1;
+        for (int a = 0; a < na && atom[s] < 0; ++a)
+            if (std::memcmp(pr.shells[s].A, nuclei->positions + 3 * a, 3 * sizeof(double)) == 0) atom[s] = a;
+        if (atom[s] < 0) return ctx->fail(FOCK_INVALID_ARG, "a basis function centre is not on a nucleus");
+    }
+    const size_t bytes = (size_t)n * n * sizeof(double);
+    DevBuf dP, dC, dE, dW, dG, dAtom;
+    HIPCHK(dP.ensure(bytes));
+    HIPCHK(dC.ensure(bytes));
+    HIPCHK(dE.ensure(n * sizeof(double)));
+    HIPCHK(dW.ensure(bytes));
+    HIPCHK(dG.ensure(3 * na * sizeof(double)));
+    HIPCHK(dAtom.ensure(ns * sizeof(int)));
+    // P and W are symmetric: the column-major inputs serve as row-major
+    HIPCHK(hipMemcpyAsync(dP.p, scf->density, bytes, hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dC.p, scf->coefficients, bytes, hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dE.p, scf->orbital_energies, n * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dAtom.p, atom.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(fill_f64(3 * na, 0.0, dG.as<double>(), S(ctx)));
+    HIPCHK(energy_weighted_density(n, nocc, dC.as<double>(), dE.as<double>(), dW.as<double>(), S(ctx)));
+    // one-electron part
+    BasisDev bd;
+    STCHK(upload_one_electron(ctx, basis, nuclei, pr.shells, bd));
+    for (int c = 0; c < 3; ++c) {
+        GradOneArgs g{};
+        g.nbf = n;
+        g.natoms = na;
+        g.npairs = bd.npairs_cls[c];
+        g.pair_a = bd.pair_a.as<int>() + bd.off_cls[c];
+        g.pair_b = bd.pair_b.as<int>() + bd.off_cls[c];
+        g.shell_f0 = bd.shell_f0.as<int>();
+        g.shell_atom = dAtom.as<int>();
+        g.centers = bd.centers.as<double>();
+        g.prim_off = bd.prim_off.as<int>();
+        g.exps = bd.exps.as<double>();
+        g.coefs = bd.coefs.as<double>();
+        g.nuc_pos = bd.nuc_pos.as<double>();
+        g.nuc_charge = bd.nuc_q.as<double>();
+        g.P = dP.as<double>();
+        g.W = dW.as<double>();
+        g.bt = ctx->bt;
+        g.grad = dG.as<double>();
+        HIPCHK(launch_grad_one(c, g, S(ctx)));
+    }
+    // two-electron part over the unique quartets of the Fock build
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        GradEriArgs g{};
+        g.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
+        g.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
+        g.pairs = pr.d_pairs.as<PairInfo>();
+        g.prims = pr.d_prims64.as<PrimPair<double>>();
+        g.prim_ab = pr.d_prim_ab.as<double2>();
+        g.pair_shell = pr.d_pair_shell.as<int2>();
+        g.shell_atom = dAtom.as<int>();
+        g.P = dP.as<double>();
+        g.nbf = n;
+        g.natoms = na;
+        g.bt = ctx->bt;
+        g.grad = dG.as<double>();
+        HIPCHK(launch_grad_eri(cls, g, S(ctx)));
+    }
+    std::vector<double> gh(3 * na);
+    HIPCHK(hipMemcpyAsync(gh.data(), dG.p, 3 * na * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    // nuclear repulsion (scf/src/lib.rs:102-116): d/dR_i sum_{i<j} Z_i Z_j / |R_j - R_i|
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < na; ++j) {
+            if (i == j) continue;
+            const double *A = nuclei->positions + 3 * i, *B = nuclei->positions + 3 * j;
+            const double d[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+            const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const double f = -nuclei->charges[i] * nuclei->charges[j] / (r * r * r);
+            for (int k = 0; k < 3; ++k) gh[3 * i + k] += f * d[k];
+        }
+    std::memcpy(gradient, gh.data(), 3 * na * sizeof(double));
     return FOCK_OK;
 }
 

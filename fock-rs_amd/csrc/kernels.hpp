@@ -125,8 +125,38 @@ struct PrimCountArgs {
     unsigned long long *out;
 };
 
+// nuclear gradient, two-electron part: one lane per unique shell quartet of the task lists
+struct GradEriArgs {
+    const Task *tasks;
+    int ntasks;
+    const PairInfo *pairs;
+    const PrimPair<double> *prims;
+    const double2 *prim_ab;  // exponents (alpha_a, alpha_b) of every primitive pair
+    const int2 *pair_shell;
+    const int *shell_atom;
+    const double *P;
+    int nbf, natoms;
+    BoysTables bt;
+    double *grad;  // [natoms][3], accumulated
+};
+// nuclear gradient, one-electron part: one thread per shell pair of one pair class
+struct GradOneArgs {
+    int nbf, natoms, npairs;
+    const int *pair_a, *pair_b, *shell_f0, *shell_atom;
+    const double *centers;
+    const int *prim_off;
+    const double *exps, *coefs;
+    const double *nuc_pos, *nuc_charge;
+    const double *P, *W;
+    BoysTables bt;
+    double *grad;
+};
+
 hipError_t launch_eri_tasks(int cls, int mode, const JKArgs &a, int policy, bool refscreen, bool fp32,
                             hipStream_t s);
+hipError_t launch_grad_eri(int cls, const GradEriArgs &a, hipStream_t s);
+hipError_t launch_grad_one(int pair_class, const GradOneArgs &a, hipStream_t s);
+hipError_t energy_weighted_density(int n, int nocc, const double *C, const double *eps, double *W, hipStream_t s);
 hipError_t launch_prim_diag(int pair_class, const PrimDiagArgs &a, hipStream_t s);
 hipError_t launch_prim_count(const PrimCountArgs &a, hipStream_t s);
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);

@@ -38,7 +38,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 
 EXPORTED = [
     "fock_version", "fock_ctx_create", "fock_ctx_destroy", "fock_ctx_last_error", "fock_ctx_set_stream",
-    "fock_scf_opts_default", "fock_scf_run", "fock_boys", "fock_eri_dense", "fock_one_electron",
+    "fock_scf_opts_default", "fock_scf_run", "fock_rhf_gradient", "fock_boys", "fock_eri_dense", "fock_one_electron",
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
     "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
 ]
@@ -112,6 +112,7 @@ def load(path: Optional[str] = None):
         "fock_ctx_set_stream": (ctypes.c_int, [vp, vp]),
         "fock_scf_opts_default": (None, [ctypes.POINTER(ScfOpts)]),
         "fock_scf_run": (ctypes.c_int, [vp, bp, np_, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfOut)]),
+        "fock_rhf_gradient": (ctypes.c_int, [vp, bp, np_, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfOut), _dp]),
         "fock_boys": (ctypes.c_int, [vp, _dp, i64, i32, i32, _dp]),
         "fock_eri_dense": (ctypes.c_int, [vp, bp, i32, i32, d, _dp]),
         "fock_one_electron": (ctypes.c_int, [vp, bp, np_, i32, _dp, _dp, _dp]),
@@ -293,6 +294,26 @@ class Context:
             "n_basis": out.n_basis, "last_error": out.last_error, "history": history,
             "scf_wall_ms": out.scf_wall_ms,
         }
+
+    def rhf_gradient(self, fb, fn, opts: ScfOpts, result) -> np.ndarray:
+        """dE/dR (Eh/Bohr, shape (n_atoms, 3)) at a converged scf_run result."""
+        n = fb.n
+        C = np.ascontiguousarray(np.asarray(result["coefficients"], dtype=np.float64).T).ravel()  # column-major
+        P = np.ascontiguousarray(np.asarray(result["density"], dtype=np.float64).T).ravel()
+        eps = np.ascontiguousarray(result["orbital_energies"], dtype=np.float64)
+        out = ScfOut()
+        out.coefficients = _d(C)
+        out.density = _d(P)
+        out.orbital_energies = _d(eps)
+        out.n_electrons = int(result["n_electrons"])
+        out.n_basis = n
+        na = len(fn.charges)
+        g = np.zeros(3 * na)
+        bd, nd = basis_desc(fb), nuclei_desc(fn)
+        st = self.lib.fock_rhf_gradient(self.ptr, ctypes.byref(bd), ctypes.byref(nd), ctypes.byref(opts),
+                                        ctypes.byref(out), _d(g))
+        self.check(st)
+        return g.reshape(na, 3)
 
 
 def default_opts(**kw) -> ScfOpts:

@@ -203,6 +203,28 @@ def flatten(atoms: Iterable[Atom]) -> Tuple[FlatBasis, FlatNuclei]:
     return fb, fn
 
 
+def at_positions(fb: "FlatBasis", fn: "FlatNuclei", positions) -> Tuple["FlatBasis", "FlatNuclei"]:
+    """The same molecule with nucleus i at positions[i] (Bohr); every basis function centred on a
+    nucleus moves with it (the basis functions of an Atom share its position,
+    basis_set/src/lib.rs:79-119)."""
+    positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+    if positions.shape != fn.positions.shape:
+        raise ValueError("one position per nucleus expected")
+    centers = fb.centers.copy()
+    for i in range(len(fn.charges)):
+        on = np.all(fb.centers == fn.positions[i], axis=1)
+        centers[on] = positions[i]
+    return (FlatBasis(centers, fb.angular, fb.prim_offset, fb.exponents, fb.coefficients),
+            FlatNuclei(np.ascontiguousarray(positions), fn.charges))
+
+
+def displaced(fb: "FlatBasis", fn: "FlatNuclei", atom: int, axis: int, h: float):
+    """Nucleus `atom` (and its basis functions) moved by h Bohr along `axis`."""
+    pos = fn.positions.copy()
+    pos[atom, axis] += h
+    return at_positions(fb, fn, pos)
+
+
 def n_electrons(atoms: Sequence[Atom], charge: int = 0) -> int:
     """scf/src/lib.rs:91-94."""
     return sum(a.electron_count() for a in atoms) - charge

@@ -169,6 +169,16 @@ void fock_scf_opts_default(fock_scf_opts *opts);
 fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
                          const fock_scf_opts *opts, fock_scf_out *out);
 
+/* Analytic RHF nuclear gradient dE/dR in Eh/Bohr, gradient[3 * atom + axis], at a converged
+ * fock_scf_run result (density, coefficients, orbital energies and n_electrons of `scf`). The
+ * reference has no gradient (differentiation form): dE/dR = sum P dH - sum W dS +
+ * 1/2 sum (P P - exchange) d(ab|cd) + dE_nuc, W = 2 sum_occ eps_i C_i C_i^T, with the energy of
+ * scf/src/lib.rs:223 and the nuclear repulsion of :102-116. Accurate integral policy only (the
+ * reference Boys table is not differentiable); opts->screen / schwarz_threshold select the
+ * quartets as in the SCF. Every basis-function centre must coincide with a nucleus. */
+fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
+                              const fock_scf_opts *opts, const fock_scf_out *scf, double *gradient);
+
 /* --------------------------------------------------------- operator surfaces */
 
 /* Boys function F_n(T) for nT arguments; replaces boys::boys (boys/src/lib.rs:58).

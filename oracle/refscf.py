@@ -218,3 +218,24 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
         elif not is_normal(err):
             return None
     return None
+
+
+def fd_gradient(fb, fn, h: float = 1e-4, epsilon: float = 1e-13, max_iters: int = 400, **kw) -> np.ndarray:
+    """Central finite differences of the accurate (unscreened, oracle64) SCF energy, Eh/Bohr,
+    shape (n_atoms, 3): the independent check of the analytic gradient (step h in Bohr). The
+    energy expression 1/2 tr(P_new (2H + G(P_old))) is first order in the SCF error, so the SCF is
+    converged to epsilon = 1e-13 (energy noise ~1e-12, i.e. ~1e-8 in the difference quotient)."""
+    from fock_amd.molecule import displaced
+    na = len(fn.charges)
+    g = np.zeros((na, 3))
+    for a in range(na):
+        for x in range(3):
+            e = []
+            for s in (+1.0, -1.0):
+                b2, n2 = displaced(fb, fn, a, x, s * h)
+                r = try_scf(b2, n2, max_iters, epsilon, 0, accurate=True, **kw)
+                if r is None:
+                    raise RuntimeError("SCF failed at a displaced geometry")
+                e.append(r.total_energy)
+            g[a, x] = (e[0] - e[1]) / (2.0 * h)
+    return g

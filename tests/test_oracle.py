@@ -300,3 +300,31 @@ def test_small_fixtures_regenerate_from_oracle():
     assert r.total_energy == pytest.approx(g["w7_ref_energy"][0], abs=1e-10)
     packed, _ = refscf.eri_packed(fb, False)
     assert np.abs(refscf.dense_from_packed(fb.n, packed) - g["w7_ref_eri"]).max() == 0.0
+
+
+def test_fd_gradient_invariances():
+    """The oracle's finite-difference gradient: translation invariance (forces sum to zero) and the
+    mirror symmetry of water (y -> -y swaps the hydrogens), for H2O/STO-3G and H2O/6-31G."""
+    from oracle import refscf
+    for atoms in (m.h2o_sto3g(), m.build(m.water_geometry(), "6-31G")):
+        fb, fn = m.flatten(atoms)
+        g = refscf.fd_gradient(fb, fn)
+        assert np.abs(g.sum(axis=0)).max() < 2e-8
+        assert abs(g[1, 1] + g[2, 1]) < 2e-8 and abs(g[1, 2] - g[2, 2]) < 2e-8
+        assert np.abs(g[:, 0]).max() < 1e-9            # planar molecule in yz: no x force
+        assert np.abs(g).max() > 1e-3                  # not at the minimum
+
+
+def test_fd_gradient_h2_matches_energy_curve():
+    """H2/STO-3G: the force on atom 2 along the bond is dE/dR of the energy curve E(R)."""
+    from oracle import refscf
+    b = m.BasisSet.load("STO-3G")
+    R = 1.4
+    fb, fn = m.flatten([b.get((0, 0, 0), 1), b.get((0, 0, R), 1)])
+    g = refscf.fd_gradient(fb, fn)
+    e = []
+    for r in (R + 1e-3, R - 1e-3):
+        fb2, fn2 = m.flatten([b.get((0, 0, 0), 1), b.get((0, 0, r), 1)])
+        e.append(refscf.try_scf(fb2, fn2, 200, 1e-13, 0, accurate=True).total_energy)
+    dEdR = (e[0] - e[1]) / 2e-3
+    assert abs(g[1, 2] - dEdR) < 1e-6 and abs(g[0, 2] + dEdR) < 1e-6
