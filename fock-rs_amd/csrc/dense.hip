@@ -314,6 +314,24 @@ __global__ void scatter_jk_kernel(int npair, const int2 *__restrict__ pair_shell
             v = 0.0;
         }
 }
+__global__ void scatter_jk_batch_kernel(const ScatterRec *__restrict__ recs) {
+    const ScatterRec r = recs[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= r.npair) return;
+    const int2 ab = r.pair_shell[j];
+    const int fa = r.shell_f0[ab.x], fb = r.shell_f0[ab.y], na = r.shell_nf[ab.x], nb = r.shell_nf[ab.y];
+    for (int c = 0; c < na; ++c)
+        for (int d = 0; d < nb; ++d) {
+            double &v = r.Jk[(size_t)(c * nb + d) * r.npair + j];
+            r.J[(size_t)(fa + c) * r.n + (fb + d)] += v;
+            v = 0.0;
+        }
+}
+hipError_t scatter_jk_batch(int nrec, const ScatterRec *recs, int max_npair, hipStream_t s) {
+    if (nrec <= 0 || max_npair <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_jk_batch_kernel, dim3((max_npair + 255) / 256, nrec), dim3(256), 0, s, recs);
+    return hipGetLastError();
+}
 hipError_t scatter_jk(int npair, const int2 *pair_shell, const int *shell_f0, const int *shell_nf, double *Jk,
                       double *J, int n, hipStream_t s) {
     if (npair <= 0) return hipSuccess;

@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void prim_count_kernel(PrimCountArgs args) {
 // workgroup accumulates those rows in LDS with LDS atomics and flushes each nonzero entry once;
 // J[a][b] is reduced over the workgroup; only J[c][d] is a per-lane global atomic.
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
-__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
+__global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
     constexpr int L = LA + LB + LC + LD;
@@ -397,7 +397,33 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | J_ab partials of the 4 waves |
     // the block's tasks | its kept ket pairs
     extern __shared__ __align__(32) unsigned char lds_bytes[];
-    const int n = args.nbf;
+    JKArgs args = args_in;
+    Block blk_;
+    if (args_in.batch) {
+        // batched launch: this block belongs to one molecule's record
+        const int2 bb = args_in.bblocks[blockIdx.x];
+        const BatchRec r = args_in.batch[bb.x];
+        if (!r.active) return;  // the same for every thread of the block
+        args.tasks = r.tasks;
+        args.pairs = r.pairs;
+        args.prims = r.prims;
+        args.prim_q = r.prim_q;
+        args.soa = r.soa;
+        args.soa_pairs = r.soa_pairs;
+        args.soa_stride = r.soa_stride;
+        args.D = r.D;
+        args.P = r.P;
+        args.J = r.J;
+        args.K = r.K;
+        args.Jk = r.Jk;
+        args.npair = r.npair;
+        args.nbf = r.nbf;
+        blk_ = r.blocks[bb.y];
+    } else {
+        blk_ = args.blocks[blockIdx.x];
+    }
+    const Block blk = blk_;
+    const int n = args.nbf;  // LDS is sized for the largest basis of the launch
     PrimPair<Real> *bra_pp = reinterpret_cast<PrimPair<Real> *>(lds_bytes);  // [MAX_PRIM_PAIRS]
     float *bra_q = reinterpret_cast<float *>(bra_pp + MAX_PRIM_PAIRS);       // [MAX_PRIM_PAIRS]
     Real *slice = reinterpret_cast<Real *>(bra_q + MAX_PRIM_PAIRS);          // [BOYS_GRID_N][COLS]
@@ -411,7 +437,6 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args) {
     int *s_stride = reinterpret_cast<int *>(s_kx + BLOCK_TASKS * 64);      // [BLOCK_TASKS] bucket sizes
     unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_stride + 2 * BLOCK_TASKS);  // [BLOCK_TASKS]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const Block blk = args.blocks[blockIdx.x];
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
     const double *__restrict__ P = args.P;

@@ -17,6 +17,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <memory>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -413,6 +415,11 @@ extern "C" fock_status fock_boys(fock_ctx *ctx, const double *T, int64_t nT, int
 
 // ====================================================================== J/K preparation
 
+namespace {
+fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *basis, int32_t policy, int32_t screen,
+                         double tau, int32_t precision);
+}  // namespace
+
 extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *basis, int32_t policy, int32_t screen,
                                        double tau, int32_t precision) {
     if (!ctx) return FOCK_INVALID_ARG;
@@ -423,7 +430,13 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
         return ctx->fail(FOCK_INVALID_ARG, "mixed precision is defined for the accurate policy only");
     if (screen == FOCK_SCREEN_SCHWARZ && !(tau >= 0.0)) return ctx->fail(FOCK_INVALID_ARG, "bad Schwarz threshold");
     HIPCHK(hipSetDevice(ctx->device));
-    Prepared &pr = ctx->jk;
+    return prepare_into(ctx, ctx->jk, basis, policy, screen, tau, precision);
+}
+
+namespace {
+// Shell pairs, primitive-pair data, screen lists, tasks and blocks of one basis into pr.
+fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *basis, int32_t policy, int32_t screen,
+                         double tau, int32_t precision) {
     pr.valid = false;
     STCHK(build_shells(ctx, basis, pr.shells));
     const int n = basis->n_functions;
@@ -805,8 +818,6 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
     pr.valid = true;
     return FOCK_OK;
 }
-
-namespace {
 
 // Primitive quartets the primitive screen keeps, per class (statistics and the algorithmic flop
 // count of the accurate policy); one pass over the task lists.
@@ -1315,6 +1326,368 @@ extern "C" fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const do
     HIPCHK(hipMemcpyAsync(out, dO.p, len * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
     HIPCHK(hipStreamSynchronize(S(ctx)));
     return FOCK_OK;
+}
+
+namespace {
+
+// One molecule of a batched SCF: the single-molecule driver's state, advanced in lockstep.
+struct BatchMol {
+    const fock_basis_desc *basis = nullptr;
+    const fock_nuclei_desc *nuc = nullptr;
+    fock_scf_out *out = nullptr;
+    Prepared pr;
+    int n = 0, nocc = 0, group = 0, gidx = 0;
+    double e_nuc = 0.0;
+    DevBuf S, T, V, H, X, U, T1, T2, P, G, F, Fx, C, errs, focks, dots, coef, slots;
+    std::deque<int> ring;                 // stored (error, Fock) slots, oldest first
+    std::deque<std::vector<double>> dotm;  // <e_i, e_j> of the stored slots
+    int next_slot = 0;
+    double err = 0.0, e_elec = 0.0;
+    int iter = 0;
+    bool done = false;
+    int32_t status = FOCK_NOT_CONVERGED;
+};
+
+// Batched eigenproblems of equal size: A (count x n x n, symmetric) -> eigenvectors R[k][i] per
+// matrix (row-major rows), W ascending.
+fock_status eig_batch(fock_ctx *ctx, int n, int count, double *A, double *W, double *work, int *info) {
+    rocblas_status st = rocsolver_dsyevd_strided_batched(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, A,
+                                                         n, (rocblas_stride)n * n, W, n, work, n, info, count);
+    if (st != rocblas_status_success)
+        return ctx->fail(FOCK_HIP_ERROR, "rocsolver_dsyevd_strided_batched failed with status " + std::to_string((int)st));
+    return FOCK_OK;
+}
+
+}  // namespace
+
+extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fock_basis_desc *bases,
+                                          const fock_nuclei_desc *nuclei, const fock_scf_opts *opts_in,
+                                          fock_scf_out *outs, int32_t *status) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (count <= 0 || !bases || !nuclei || !outs) return ctx->fail(FOCK_INVALID_ARG, "empty or null batch");
+    fock_scf_opts opts;
+    if (opts_in) opts = *opts_in;
+    else fock_scf_opts_default(&opts);
+    if (opts.precision != FOCK_PRECISION_FP64 || opts.incremental)
+        return ctx->fail(FOCK_UNSUPPORTED, "the batched SCF runs the FP64 form");
+    if (opts.initial_density) return ctx->fail(FOCK_UNSUPPORTED, "no initial density in the batched SCF");
+    if (opts.diis_window < 1 || opts.diis_window > 16 || opts.diis_start < 1)
+        return ctx->fail(FOCK_INVALID_ARG, "DIIS window must be 1..16 and start >= 1");
+    const auto t_start = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = S(ctx);
+    std::vector<std::unique_ptr<BatchMol>> mols;
+    int nmax = 0;
+    for (int m = 0; m < count; ++m) {
+        auto bm = std::make_unique<BatchMol>();
+        bm->basis = bases + m;
+        bm->nuc = nuclei + m;
+        bm->out = outs + m;
+        const fock_nuclei_desc *nu = bm->nuc;
+        double zsum = 0.0;
+        for (int a = 0; a < nu->n_atoms; ++a) zsum += nu->charges[a];
+        const int ne = (int)std::llround(zsum) - opts.molecule_charge;
+        bm->n = bm->basis->n_functions;
+        if (ne < 0 || ne / 2 > bm->n) return ctx->fail(FOCK_INVALID_ARG, "bad electron count");
+        bm->nocc = ne / 2;
+        bm->out->n_electrons = ne;
+        bm->out->n_basis = bm->n;
+        for (int i = 0; i < nu->n_atoms; ++i)
+            for (int j = i + 1; j < nu->n_atoms; ++j) {
+                const double *A = nu->positions + 3 * i, *B = nu->positions + 3 * j;
+                const double d[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+                bm->e_nuc += nu->charges[i] * nu->charges[j] / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            }
+        STCHK(prepare_into(ctx, bm->pr, bm->basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, 0));
+        nmax = std::max(nmax, bm->n);
+        mols.push_back(std::move(bm));
+    }
+    if (!blocked_fits(ctx, nmax, false)) return ctx->fail(FOCK_UNSUPPORTED, "largest basis too large for a batch");
+    // size groups: molecules of equal n share one batched eigensolver call
+    std::map<int, std::vector<int>> groups;
+    for (int m = 0; m < count; ++m) groups[mols[m]->n].push_back(m);
+    struct Group {
+        int n;
+        std::vector<int> members;
+        DevBuf A, W, work, info;
+    };
+    std::vector<std::unique_ptr<Group>> gr;
+    for (auto &kv : groups) {
+        auto gp = std::make_unique<Group>();
+        Group &g = *gp;
+        g.n = kv.first;
+        g.members = kv.second;
+        const size_t c = g.members.size(), nn = (size_t)g.n * g.n;
+        HIPCHK(g.A.ensure(c * nn * sizeof(double)));
+        HIPCHK(g.W.ensure(c * g.n * sizeof(double)));
+        HIPCHK(g.work.ensure(c * g.n * sizeof(double)));
+        HIPCHK(g.info.ensure(c * sizeof(int)));
+        for (size_t i = 0; i < c; ++i) {
+            mols[g.members[i]]->group = (int)gr.size();
+            mols[g.members[i]]->gidx = (int)i;
+        }
+        gr.push_back(std::move(gp));
+    }
+    auto gA = [&](BatchMol &b) { return gr[b.group]->A.as<double>() + (size_t)b.gidx * b.n * b.n; };
+    auto gW = [&](BatchMol &b) { return gr[b.group]->W.as<double>() + (size_t)b.gidx * b.n; };
+    // per-molecule matrices, S/T/V, H
+    for (auto &bp : mols) {
+        BatchMol &b = *bp;
+        const size_t nn = (size_t)b.n * b.n, bytes = nn * sizeof(double);
+        for (DevBuf *d : {&b.S, &b.T, &b.V, &b.H, &b.X, &b.U, &b.T1, &b.T2, &b.P, &b.G, &b.F, &b.Fx, &b.C})
+            HIPCHK(d->ensure(bytes));
+        HIPCHK(b.errs.ensure(16 * bytes));
+        HIPCHK(b.focks.ensure(16 * bytes));
+        HIPCHK(b.dots.ensure(17 * sizeof(double)));
+        HIPCHK(b.coef.ensure(16 * sizeof(double)));
+        HIPCHK(b.slots.ensure(16 * sizeof(int)));
+        HIPCHK(fill_f64(16 * (int64_t)nn, 0.0, b.errs.as<double>(), st));
+        BasisDev bd;
+        STCHK(upload_one_electron(ctx, b.basis, b.nuc, b.pr.shells, bd));
+        STCHK(enqueue_one_electron(ctx, b.basis, b.nuc, bd, opts.integral_policy, b.S.as<double>(), b.T.as<double>(),
+                                   b.V.as<double>()));
+        HIPCHK(add_f64((int64_t)nn, b.T.as<double>(), b.V.as<double>(), b.H.as<double>(), st));
+        HIPCHK(hipStreamSynchronize(st));  // bd goes out of scope
+        HIPCHK(copy_f64((int64_t)nn, b.S.as<double>(), gA(b), st));
+    }
+    // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146), eigenproblems batched
+    for (auto &g : gr) STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
+                                       g->work.as<double>(), g->info.as<int>()));
+    for (auto &bp : mols) {
+        BatchMol &b = *bp;
+        const int n = b.n;
+        double *T1 = b.T1.as<double>(), *T2 = b.T2.as<double>(), *U = b.U.as<double>();
+        HIPCHK(transpose_f64(n, gA(b), U, st));
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.S.as<double>(), n, U, n, 0.0, T2, n, st));
+        HIPCHK(gemm_f64(true, false, n, n, n, 1.0, U, n, T2, n, 0.0, T1, n, st));
+        HIPCHK(scale_columns(n, U, T1, n, T2, st));
+        HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U, n, 0.0, b.X.as<double>(), n, st));
+    }
+    // F' = X^T M X for every molecule into its group slot, batched eigensolve, C = X R'^T, P
+    auto diagonalize_all = [&](auto fock_of) -> fock_status {
+        for (auto &bp : mols) {
+            BatchMol &b = *bp;
+            if (b.done) continue;
+            const int n = b.n;
+            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, fock_of(b), n, b.X.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
+            HIPCHK(gemm_f64(true, false, n, n, n, 1.0, b.X.as<double>(), n, b.T1.as<double>(), n, 0.0, gA(b), n, st));
+        }
+        for (auto &g : gr) STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
+                                           g->work.as<double>(), g->info.as<int>()));
+        for (auto &bp : mols) {
+            BatchMol &b = *bp;
+            if (b.done) continue;
+            const int n = b.n;
+            HIPCHK(gemm_f64(false, true, n, n, n, 1.0, b.X.as<double>(), n, gA(b), n, 0.0, b.C.as<double>(), n, st));
+            HIPCHK(density_from_coeffs(n, b.nocc, b.C.as<double>(), b.P.as<double>(), st));
+        }
+        return FOCK_OK;
+    };
+    // GWH guess (scf/src/lib.rs:150-162)
+    for (auto &bp : mols) HIPCHK(gwh_guess(bp->n, bp->S.as<double>(), bp->H.as<double>(), bp->F.as<double>(), st));
+    STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.F.as<double>(); }));
+
+    // batched Fock-build tables: one record per (molecule, class), blocks of every molecule
+    // sorted by cost (descending) per class
+    std::vector<BatchRec> recs((size_t)count * NUM_CLASSES);
+    std::vector<std::vector<int2>> bbl(NUM_CLASSES);
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        std::vector<std::pair<int, int2>> cb;  // (cost, (record, block))
+        for (int m = 0; m < count; ++m) {
+            Prepared &pr = mols[m]->pr;
+            const int nb = pr.block_off[cls + 1] - pr.block_off[cls];
+            std::vector<Block> hb(nb);
+            if (nb) HIPCHK(hipMemcpy(hb.data(), pr.d_blocks.as<Block>() + pr.block_off[cls], nb * sizeof(Block),
+                                     hipMemcpyDeviceToHost));
+            for (int i = 0; i < nb; ++i) cb.push_back({hb[i].cost, make_int2(m * NUM_CLASSES + cls, i)});
+        }
+        std::stable_sort(cb.begin(), cb.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        for (auto &e : cb) bbl[cls].push_back(e.second);
+    }
+    DevBuf d_recs, d_bbl[NUM_CLASSES], d_scat;
+    HIPCHK(d_recs.ensure(recs.size() * sizeof(BatchRec)));
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        HIPCHK(d_bbl[cls].ensure(std::max<size_t>(bbl[cls].size(), 1) * sizeof(int2)));
+        HIPCHK(hipMemcpy(d_bbl[cls].p, bbl[cls].data(), bbl[cls].size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    std::vector<ScatterRec> scat(count);
+    int max_npair = 0;
+    for (int m = 0; m < count; ++m) {
+        Prepared &pr = mols[m]->pr;
+        const int np = (int)pr.pairs.size();
+        max_npair = std::max(max_npair, np);
+        scat[m] = ScatterRec{np, pr.n, pr.d_pair_shell.as<int2>(), pr.d_shell_f0.as<int>(), pr.d_shell_nf.as<int>(),
+                             pr.d_Jk.as<double>(), pr.d_Jb.as<double>()};
+    }
+    HIPCHK(d_scat.ensure(count * sizeof(ScatterRec)));
+    HIPCHK(hipMemcpy(d_scat.p, scat.data(), count * sizeof(ScatterRec), hipMemcpyHostToDevice));
+    auto upload_records = [&]() -> fock_status {
+        for (int m = 0; m < count; ++m) {
+            Prepared &pr = mols[m]->pr;
+            for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+                BatchRec &r = recs[(size_t)m * NUM_CLASSES + cls];
+                r.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
+                r.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
+                r.pairs = pr.d_pairs.as<PairInfo>();
+                r.prims = pr.d_prims64.p;
+                r.prim_q = pr.prim_screen ? pr.d_prim_q.as<float>() : nullptr;
+                r.soa = pr.d_soa64.p;
+                r.soa_pairs = pr.d_soa_pairs.as<PairInfo>();
+                r.soa_stride = pr.d_soa_stride.as<int>();
+                r.D = pr.d_D.as<double>();
+                r.P = mols[m]->P.as<double>();
+                r.J = pr.d_Jb.as<double>();
+                r.K = pr.d_Kb.as<double>();
+                r.Jk = pr.d_Jk.as<double>();
+                r.npair = (int)pr.pairs.size();
+                r.nbf = pr.n;
+                r.active = mols[m]->done ? 0 : 1;
+            }
+        }
+        HIPCHK(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * sizeof(BatchRec), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));  // recs is rewritten on the host next time
+        return FOCK_OK;
+    };
+    STCHK(upload_records());
+    const bool refscreen = opts.screen == FOCK_SCREEN_REF_BF_PAIR;
+    std::vector<double> hostd((size_t)count * 18);
+    int active = count;
+    for (int iter = 0; iter <= opts.max_iters && active > 0; ++iter) {
+        // G(P), F = H + G for every active molecule: one launch per class for the whole batch
+        for (auto &bp : mols) {
+            if (bp->done) continue;
+            const int64_t nn = (int64_t)bp->n * bp->n;
+            HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Jb.as<double>(), st));
+            HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Kb.as<double>(), st));
+        }
+        for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+            if (bbl[cls].empty()) continue;
+            JKArgs a{};
+            a.nbf = nmax;
+            a.nblocks = (int)bbl[cls].size();
+            a.bt = ctx->bt;
+            a.dthr = mols[0]->pr.dthr;
+            a.prim_thr = PRIM_THRESHOLD;
+            a.batch = d_recs.as<BatchRec>() + 0;
+            a.bblocks = d_bbl[cls].as<int2>();
+            HIPCHK(launch_eri_blocks(cls, a, opts.integral_policy, refscreen, false, st));
+        }
+        HIPCHK(scatter_jk_batch(count, d_scat.as<ScatterRec>(), max_npair, st));
+        for (auto &bp : mols) {
+            BatchMol &b = *bp;
+            if (b.done) continue;
+            const int n = b.n;
+            const int64_t nn = (int64_t)n * n;
+            HIPCHK(fock_from_jk(n, b.H.as<double>(), b.pr.d_Jb.as<double>(), b.pr.d_Kb.as<double>(), b.G.as<double>(),
+                                b.F.as<double>(), 0.5, st));
+            // DIIS error FPS - SPF (scf/src/lib.rs:189-192) into the ring, dots with every slot
+            const int slot = b.next_slot;
+            double *e_slot = b.errs.as<double>() + (int64_t)slot * nn;
+            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.F.as<double>(), n, b.P.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
+            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.T1.as<double>(), n, b.S.as<double>(), n, 0.0, b.T2.as<double>(), n, st));
+            HIPCHK(fill_f64(1, 0.0, b.dots.as<double>() + 16, st));
+            HIPCHK(antisym_maxabs(n, b.T2.as<double>(), e_slot, b.dots.as<double>() + 16, st));
+            HIPCHK(copy_f64(nn, b.F.as<double>(), b.focks.as<double>() + (int64_t)slot * nn, st));
+            HIPCHK(frob_dots(nn, e_slot, b.errs.as<double>(), nn, 16, b.dots.as<double>(), st));
+            HIPCHK(hipMemcpyAsync(hostd.data() + (size_t)(&bp - &mols[0]) * 18, b.dots.p, 17 * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        for (int m = 0; m < count; ++m) {
+            BatchMol &b = *mols[m];
+            if (b.done) continue;
+            const int n = b.n;
+            const int64_t nn = (int64_t)n * n;
+            const double *hd = hostd.data() + (size_t)m * 18;
+            const int slot = b.next_slot;
+            b.next_slot = (b.next_slot + 1) % 16;
+            if ((int)b.ring.size() == opts.diis_window) {
+                b.ring.pop_front();
+                b.dotm.pop_front();
+                for (auto &row : b.dotm) row.erase(row.begin());
+            }
+            b.ring.push_back(slot);
+            const int mm = (int)b.ring.size();
+            std::vector<double> nd(mm);
+            for (int i = 0; i < mm; ++i) nd[i] = hd[b.ring[i]];
+            for (int i = 0; i < mm - 1; ++i) b.dotm[i].push_back(nd[i]);
+            b.dotm.push_back(nd);
+            b.err = hd[16];
+            const double *Fuse = b.focks.as<double>() + (int64_t)slot * nn;
+            if (mm >= opts.diis_start) {
+                std::vector<double> Bm((size_t)mm * mm), c;
+                for (int i = 0; i < mm; ++i)
+                    for (int j = 0; j < mm; ++j) Bm[(size_t)i * mm + j] = b.dotm[i][j];
+                if (diis_solve(mm, Bm, c)) {
+                    std::vector<int> sl(b.ring.begin(), b.ring.end());
+                    HIPCHK(hipMemcpyAsync(b.coef.p, c.data(), mm * sizeof(double), hipMemcpyHostToDevice, st));
+                    HIPCHK(hipMemcpyAsync(b.slots.p, sl.data(), mm * sizeof(int), hipMemcpyHostToDevice, st));
+                    HIPCHK(lincomb(nn, b.focks.as<double>(), nn, b.slots.as<int>(), b.coef.as<double>(), mm,
+                                   b.Fx.as<double>(), st));
+                    HIPCHK(hipStreamSynchronize(st));  // c, sl are host temporaries
+                    Fuse = b.Fx.as<double>();
+                }
+            }
+            if (opts.level_shift > 0.0 && b.err > opts.level_shift_until) {
+                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.S.as<double>(), n, b.P.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
+                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.T1.as<double>(), n, b.S.as<double>(), n, 0.0, b.T2.as<double>(), n, st));
+                HIPCHK(level_shift_fock(nn, Fuse, b.S.as<double>(), b.T2.as<double>(), opts.level_shift, b.Fx.as<double>(), st));
+                Fuse = b.Fx.as<double>();
+            }
+            if (Fuse != b.Fx.as<double>()) HIPCHK(copy_f64(nn, Fuse, b.Fx.as<double>(), st));
+        }
+        STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.Fx.as<double>(); }));
+        // E_elec = 1/2 tr(P_new (2H + G(P_old))) (scf/src/lib.rs:223)
+        for (int m = 0; m < count; ++m) {
+            BatchMol &b = *mols[m];
+            if (b.done) continue;
+            HIPCHK(energy_trace(b.n, b.P.as<double>(), b.H.as<double>(), b.G.as<double>(), b.dots.as<double>() + 16, st));
+            HIPCHK(hipMemcpyAsync(hostd.data() + (size_t)m * 18 + 17, b.dots.as<double>() + 16, sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        bool changed = false;
+        for (int m = 0; m < count; ++m) {
+            BatchMol &b = *mols[m];
+            if (b.done) continue;
+            b.e_elec = hostd[(size_t)m * 18 + 17];
+            b.iter = iter;
+            if (b.err < opts.epsilon || iter == opts.max_iters || !is_normal(b.err)) {
+                b.status = b.err < opts.epsilon ? FOCK_OK : (is_normal(b.err) ? FOCK_NOT_CONVERGED : FOCK_DIVERGED);
+                b.done = true;
+                changed = true;
+                --active;
+                // outputs, column-major (nalgebra layout)
+                const int n = b.n;
+                const size_t bytes = (size_t)n * n * sizeof(double);
+                fock_scf_out *o = b.out;
+                o->electronic_energy = b.e_elec;
+                o->total_energy = b.e_elec + b.e_nuc;
+                o->iterations = iter;
+                o->last_error = b.err;
+                o->history_len = 0;
+                if (o->coefficients) {
+                    HIPCHK(transpose_f64(n, b.C.as<double>(), b.T1.as<double>(), st));
+                    HIPCHK(hipMemcpyAsync(o->coefficients, b.T1.p, bytes, hipMemcpyDeviceToHost, st));
+                }
+                if (o->density) HIPCHK(hipMemcpyAsync(o->density, b.P.p, bytes, hipMemcpyDeviceToHost, st));
+                if (o->orbital_energies)
+                    HIPCHK(hipMemcpyAsync(o->orbital_energies, gW(b), n * sizeof(double), hipMemcpyDeviceToHost, st));
+            }
+        }
+        if (changed) STCHK(upload_records());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    fock_status result = FOCK_OK;
+    for (int m = 0; m < count; ++m) {
+        mols[m]->out->scf_wall_ms = wall;
+        if (status) status[m] = mols[m]->status;
+        if (mols[m]->status == FOCK_DIVERGED) result = FOCK_DIVERGED;
+        else if (mols[m]->status != FOCK_OK && result == FOCK_OK) result = FOCK_NOT_CONVERGED;
+    }
+    if (result != FOCK_OK) ctx->err = "some molecules of the batch did not converge";
+    return result;
 }
 
 extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,

@@ -58,6 +58,26 @@ struct JKArgs {
     // every atomic instruction covers a few cache lines), scattered into J after the build
     double *Jk;
     int npair;
+    // batched launch (many molecules): block i of the launch is block bblocks[i].y of record
+    // bblocks[i].x, whose arrays replace the ones above; null: one molecule
+    const struct BatchRec *batch;
+    const int2 *bblocks;
+};
+// one molecule's arrays for one quartet class of a batched Fock build
+struct BatchRec {
+    const Task *tasks;
+    const Block *blocks;
+    const PairInfo *pairs;
+    const void *prims;
+    const float *prim_q;
+    const void *soa;
+    const PairInfo *soa_pairs;
+    const int *soa_stride;
+    const double *D;
+    const double *P;
+    double *J, *K, *Jk;
+    int npair, nbf;
+    int active;  // 0: the molecule has converged, its blocks return at once
 };
 constexpr int PRIM_FIELDS = 13;
 // largest primitive-pair count of a shell pair (contractions of up to 6 primitives)
@@ -92,6 +112,14 @@ hipError_t compact_tasks(const Task *tasks, int ntasks, const int *gfirst, const
                          const CompactBuffers &b, hipStream_t s);
 // Gacc += (Jb + Jb^T) - 0.5 (Kb + Kb^T)
 hipError_t accumulate_g(int n, const double *Jb, const double *Kb, double *Gacc, hipStream_t s);
+// one molecule of a batched scatter_jk
+struct ScatterRec {
+    int npair, n;
+    const int2 *pair_shell;
+    const int *shell_f0, *shell_nf;
+    double *Jk, *J;
+};
+hipError_t scatter_jk_batch(int nrec, const ScatterRec *recs, int max_npair, hipStream_t s);
 // J[f0(a) + c][f0(b) + d] += Jk[(c * nf(b) + d) * npair + j] for every pair j = (a, b), then Jk = 0
 hipError_t scatter_jk(int npair, const int2 *pair_shell, const int *shell_f0, const int *shell_nf, double *Jk,
                       double *J, int n, hipStream_t s);

@@ -38,7 +38,7 @@ _ip = ctypes.POINTER(ctypes.c_int32)
 
 EXPORTED = [
     "fock_version", "fock_ctx_create", "fock_ctx_destroy", "fock_ctx_last_error", "fock_ctx_set_stream",
-    "fock_scf_opts_default", "fock_scf_run", "fock_rhf_gradient", "fock_boys", "fock_eri_dense", "fock_one_electron",
+    "fock_scf_opts_default", "fock_scf_run", "fock_rhf_gradient", "fock_scf_batch_run", "fock_boys", "fock_eri_dense", "fock_one_electron",
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
     "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
 ]
@@ -113,6 +113,8 @@ def load(path: Optional[str] = None):
         "fock_scf_opts_default": (None, [ctypes.POINTER(ScfOpts)]),
         "fock_scf_run": (ctypes.c_int, [vp, bp, np_, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfOut)]),
         "fock_rhf_gradient": (ctypes.c_int, [vp, bp, np_, ctypes.POINTER(ScfOpts), ctypes.POINTER(ScfOut), _dp]),
+        "fock_scf_batch_run": (ctypes.c_int, [vp, ctypes.c_int32, bp, np_, ctypes.POINTER(ScfOpts),
+                                              ctypes.POINTER(ScfOut), ctypes.POINTER(ctypes.c_int32)]),
         "fock_boys": (ctypes.c_int, [vp, _dp, i64, i32, i32, _dp]),
         "fock_eri_dense": (ctypes.c_int, [vp, bp, i32, i32, d, _dp]),
         "fock_one_electron": (ctypes.c_int, [vp, bp, np_, i32, _dp, _dp, _dp]),
@@ -294,6 +296,37 @@ class Context:
             "n_basis": out.n_basis, "last_error": out.last_error, "history": history,
             "scf_wall_ms": out.scf_wall_ms,
         }
+
+    def scf_batch_run(self, molecules, opts: ScfOpts):
+        """Lockstep SCF of many (FlatBasis, FlatNuclei) molecules; returns (overall status,
+        per-molecule statuses, per-molecule results as scf_run's)."""
+        count = len(molecules)
+        bds = (BasisDesc * count)()
+        nds = (NucleiDesc * count)()
+        outs = (ScfOut * count)()
+        keep, arrays = [], []
+        for i, (fb, fn) in enumerate(molecules):
+            bd, nd = basis_desc(fb), nuclei_desc(fn)
+            keep.append((bd, nd, fb, fn))
+            bds[i], nds[i] = bd, nd
+            n = fb.n
+            C, P, eps = np.zeros(n * n), np.zeros(n * n), np.zeros(n)
+            arrays.append((C, P, eps))
+            outs[i].coefficients, outs[i].density, outs[i].orbital_energies = _d(C), _d(P), _d(eps)
+        status = (ctypes.c_int32 * count)()
+        st = self.lib.fock_scf_batch_run(self.ptr, count, bds, nds, ctypes.byref(opts), outs, status)
+        self.check(st, allow=(FOCK_OK, FOCK_NOT_CONVERGED, FOCK_DIVERGED))
+        results = []
+        for i, (fb, fn) in enumerate(molecules):
+            n = fb.n
+            C, P, eps = arrays[i]
+            o = outs[i]
+            results.append({
+                "coefficients": C.reshape(n, n).T.copy(), "density": P.reshape(n, n).T.copy(),
+                "orbital_energies": eps, "electronic_energy": o.electronic_energy, "total_energy": o.total_energy,
+                "iterations": o.iterations, "n_electrons": o.n_electrons, "n_basis": o.n_basis,
+                "last_error": o.last_error, "scf_wall_ms": o.scf_wall_ms})
+        return st, list(status), results
 
     def rhf_gradient(self, fb, fn, opts: ScfOpts, result) -> np.ndarray:
         """dE/dR (Eh/Bohr, shape (n_atoms, 3)) at a converged scf_run result."""

@@ -169,6 +169,17 @@ void fock_scf_opts_default(fock_scf_opts *opts);
 fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
                          const fock_scf_opts *opts, fock_scf_out *out);
 
+/* Many SCFs in lockstep (batching form; the reference has no batch API): molecule m is
+ * bases[m] / nuclei[m], its result outs[m] (as fock_scf_run's, history not filled) and, when
+ * status is not NULL, status[m]. Every iteration builds the Fock matrices of all unconverged
+ * molecules in one launch per quartet class (their shell-pair blocks are interleaved by cost) and
+ * solves the eigenproblems of equal-size molecules in one batched call. FP64 form only
+ * (precision FP64, no incremental builds, no initial density); the SCF schedule and options are
+ * those of fock_scf_run. Returns OK when every molecule converged, else NOT_CONVERGED or DIVERGED. */
+fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fock_basis_desc *bases,
+                               const fock_nuclei_desc *nuclei, const fock_scf_opts *opts, fock_scf_out *outs,
+                               int32_t *status);
+
 /* Analytic RHF nuclear gradient dE/dR in Eh/Bohr, gradient[3 * atom + axis], at a converged
  * fock_scf_run result (density, coefficients, orbital energies and n_electrons of `scf`). The
  * reference has no gradient (differentiation form): dE/dR = sum P dH - sum W dS +

@@ -1,0 +1,58 @@
+"""Batching form: many water clusters (H2O)_k, k uniform in 1..8, 6-31G, advanced in lockstep
+with one Fock-build launch per quartet class per iteration (fock_scf_batch_run). Each molecule is
+checked against the single-molecule path (fock_scf_run) through the same C ABI.
+"""
+import numpy as np
+import pytest
+
+from fock_amd import abi, molecule as m
+
+pytestmark = pytest.mark.gpu
+
+ROBUST = dict(diis_start=2, diis_window=10, level_shift=0.5, level_shift_until=0.05)
+
+
+def opts(**kw):
+    return abi.default_opts(max_iters=200, epsilon=1e-9, integral_policy=abi.POLICY_ACCURATE,
+                            screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12, **{**ROBUST, **kw})
+
+
+def clusters(count, seed):
+    return [m.flatten(m.build(g, "6-31G")) for g in m.water_cluster_batch(count, seed=seed)]
+
+
+def check_against_single(ctx, mols, results, o, which):
+    for i in which:
+        st, r = ctx.scf_run(mols[i][0], mols[i][1], o)
+        b = results[i]
+        assert st == abi.FOCK_OK
+        assert abs(b["total_energy"] - r["total_energy"]) < 1e-9, (i, b["total_energy"], r["total_energy"])
+        assert np.abs(b["orbital_energies"] - r["orbital_energies"]).max() < 1e-7
+        assert np.abs(b["density"] - r["density"]).max() < 1e-6
+
+
+def test_batch_matches_single_molecule_runs(ctx):
+    mols = clusters(8, seed=11)
+    o = opts()
+    st, statuses, res = ctx.scf_batch_run(mols, o)
+    assert st == abi.FOCK_OK and all(s == abi.FOCK_OK for s in statuses)
+    check_against_single(ctx, mols, res, o, range(len(mols)))
+
+
+def test_batch_64_clusters(ctx):
+    mols = clusters(64, seed=20261202)
+    sizes = sorted({fb.n for fb, _ in mols})
+    assert sizes[0] >= 13 and sizes[-1] <= 104 and len(sizes) > 4   # uneven sizes
+    o = opts()
+    st, statuses, res = ctx.scf_batch_run(mols, o)
+    assert st == abi.FOCK_OK, statuses
+    print("64 clusters: %.1f ms wall" % res[0]["scf_wall_ms"])
+    big = max(range(64), key=lambda i: mols[i][0].n)
+    small = min(range(64), key=lambda i: mols[i][0].n)
+    check_against_single(ctx, mols, res, o, [0, big, small])
+
+
+def test_batch_rejects_mixed_precision(ctx):
+    mols = clusters(2, seed=3)
+    with pytest.raises(abi.FockError):
+        ctx.scf_batch_run(mols, opts(precision=abi.PRECISION_MIXED))
