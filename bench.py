@@ -135,6 +135,43 @@ def cpu_baseline(fb, threads):
     }
 
 
+def batch_form(ctx, abi, m, np):
+    """64 water clusters (H2O)_k, k ~ U{1..8} (seed 20261202), 6-31G, accurate integrals: the
+    lockstep batched SCF against the same 64 SCFs run one after the other."""
+    mols = [m.flatten(m.build(g, "6-31G")) for g in m.water_cluster_batch(64, seed=20261202)]
+    o = abi.default_opts(max_iters=200, epsilon=SCF_EPS, integral_policy=abi.POLICY_ACCURATE,
+                         screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12, **SCF_SCHEDULE)
+    ctx.scf_batch_run(mols, o)  # warm-up (code objects, allocations)
+    t0 = time.time()
+    st, statuses, res = ctx.scf_batch_run(mols, o)
+    t_batch = time.time() - t0
+    t0 = time.time()
+    single = [ctx.scf_run(fb, fn, o)[1] for fb, fn in mols]
+    t_seq = time.time() - t0
+    return {
+        "workload": "64 (H2O)_k clusters, k in 1..8, 6-31G (n = 13..104), accurate, Schwarz 1e-12",
+        "status": abi.STATUS_NAMES[st], "batch_wall_s": t_batch, "sequential_wall_s": t_seq,
+        "speedup": t_seq / t_batch, "scf_per_s": 64 / t_batch,
+        "max_abs_dE_batch_vs_single": float(max(abs(a["total_energy"] - b["total_energy"]) for a, b in zip(res, single))),
+        "n_basis_range": [min(fb.n for fb, _ in mols), max(fb.n for fb, _ in mols)],
+    }
+
+
+def gradient_form(ctx, abi, m, np):
+    """Analytic RHF gradient of n-octane/6-31G at its converged FP64 SCF."""
+    fb, fn = m.flatten(m.octane_631g())
+    o = abi.default_opts(max_iters=200, epsilon=1e-10, integral_policy=abi.POLICY_ACCURATE,
+                         screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12, **SCF_SCHEDULE)
+    st, r = ctx.scf_run(fb, fn, o)
+    ctx.rhf_gradient(fb, fn, o, r)  # warm-up
+    t0 = time.time()
+    g = ctx.rhf_gradient(fb, fn, o, r)
+    t = time.time() - t0
+    return {"workload": "C8H18/6-31G analytic nuclear gradient (FP64)", "gradient_ms": t * 1e3,
+            "scf_wall_ms": r["scf_wall_ms"], "max_abs_force": float(np.abs(g).max()),
+            "net_force": float(np.abs(g.sum(axis=0)).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +292,11 @@ def main():
         "scf": scf, "e_oracle": e_oracle,
         "roofline": roofline,
     }
+    if not args.no_scf:
+        result["batch_form"] = batch_form(ctx, abi, m, np)
+        log("batch form: %s" % result["batch_form"])
+        result["gradient_form"] = gradient_form(ctx, abi, m, np)
+        log("gradient form: %s" % result["gradient_form"])
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
         cb = cpu_baseline(fb, threads=min(16, os.cpu_count() or 1))
