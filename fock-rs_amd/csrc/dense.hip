@@ -314,6 +314,96 @@ __global__ void scatter_jk_kernel(int npair, const int2 *__restrict__ pair_shell
             v = 0.0;
         }
 }
+// Gershgorin bounds of the spectrum of a symmetric matrix: out = {min_i (A_ii - r_i), max_i (A_ii + r_i)},
+// r_i = sum_{j != i} |A_ij|; one workgroup.
+__global__ __launch_bounds__(256) void gershgorin_kernel(int n, const double *__restrict__ A, double *__restrict__ out) {
+    __shared__ double lo[256], hi[256];
+    double l = HUGE_VAL, h = -HUGE_VAL;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double r = 0.0;
+        for (int j = 0; j < n; ++j)
+            if (j != i) r += fabs(A[(size_t)i * n + j]);
+        const double d = A[(size_t)i * n + i];
+        l = fmin(l, d - r);
+        h = fmax(h, d + r);
+    }
+    lo[threadIdx.x] = l;
+    hi[threadIdx.x] = h;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = lo[0];
+        out[1] = hi[0];
+    }
+}
+hipError_t gershgorin_bounds(int n, const double *A, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(gershgorin_kernel, dim3(1), dim3(256), 0, s, n, A, out);
+    return hipGetLastError();
+}
+
+// X = (hi I - A) / (hi - lo) with {lo, hi} = bounds (device)
+__global__ void tc2_init_kernel(int n, const double *__restrict__ A, const double *__restrict__ bounds,
+                                double *__restrict__ X) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double lo = bounds[0], hi = bounds[1];
+    X[e] = ((i == j ? hi : 0.0) - A[e]) / (hi - lo);
+}
+hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(tc2_init_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, A, bounds, X);
+    return hipGetLastError();
+}
+
+// out = {tr X, tr X2, grow}; grow = 1 when tr(2X - X2) is closer to target than tr X2; one workgroup
+__global__ __launch_bounds__(256) void trace2_kernel(int n, const double *__restrict__ X, const double *__restrict__ X2,
+                                                     double target, double *__restrict__ out) {
+    __shared__ double a[256], b[256];
+    double ta = 0.0, tb = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        ta += X[(size_t)i * n + i];
+        tb += X2[(size_t)i * n + i];
+    }
+    a[threadIdx.x] = ta;
+    b[threadIdx.x] = tb;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            a[threadIdx.x] += a[threadIdx.x + s];
+            b[threadIdx.x] += b[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = a[0];
+        out[1] = b[0];
+        out[2] = fabs(2.0 * a[0] - b[0] - target) < fabs(b[0] - target) ? 1.0 : 0.0;
+    }
+}
+hipError_t trace2(int n, const double *X, const double *X2, double target, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(trace2_kernel, dim3(1), dim3(256), 0, s, n, X, X2, target, out);
+    return hipGetLastError();
+}
+
+// X = 2 X - X2 (grow != 0) or X = X2, with the choice read from the device
+__global__ void tc2_update_kernel(int64_t len, const double *__restrict__ grow, double *__restrict__ X,
+                                  const double *__restrict__ X2) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    X[e] = *grow != 0.0 ? 2.0 * X[e] - X2[e] : X2[e];
+}
+hipError_t tc2_update(int64_t len, const double *grow, double *X, const double *X2, hipStream_t s) {
+    hipLaunchKernelGGL(tc2_update_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, grow, X, X2);
+    return hipGetLastError();
+}
+
 __global__ void scatter_jk_batch_kernel(const ScatterRec *__restrict__ recs) {
     const ScatterRec r = recs[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;

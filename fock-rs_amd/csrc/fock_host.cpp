@@ -1883,10 +1883,62 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U_, n, 0.0, X_, n, S(ctx)));    // (U diag) U^T
 
     // solve F' = X^T M X; C = X C'; P = 2 C_occ C_occ^T; eigenvalues in W_
-    auto diagonalize = [&](const double *M) -> fock_status {
+    // Density of F' = X^T M X without orbitals (accurate policy, iterations that do not exit):
+    // trace-correcting purification (Niklasson, Phys. Rev. B 66, 155115 (2002)). From
+    // X0 = (hi - F')/(hi - lo) (Gershgorin bounds) each step takes X^2 or 2X - X^2, whichever
+    // trace is closer to nocc, until tr X - tr X^2 = sum lambda (1 - lambda) vanishes; then
+    // P = 2 X Xp X^T. False (and no P) when it does not converge, e.g. without a gap at the
+    // Fermi level.
+    DevBuf dXp, dX2, dPur;
+    const bool purify_ok = opts.integral_policy == FOCK_POLICY_ACCURATE;
+    if (purify_ok) {
+        HIPCHK(dXp.ensure(bytes));
+        HIPCHK(dX2.ensure(bytes));
+        HIPCHK(dPur.ensure(4 * sizeof(double)));
+    }
+    auto purify = [&](bool &ok) -> fock_status {
+        ok = false;
+        double *Xp = dXp.as<double>(), *X2 = dX2.as<double>(), *pur = dPur.as<double>();  // pur: lo, hi | trX, trX2, grow
+        HIPCHK(gershgorin_bounds(n, T2, pur, S(ctx)));
+        HIPCHK(tc2_init(n, T2, pur, Xp, S(ctx)));
+        // steps run in chunks without host round trips (the grow/shrink choice is made on the
+        // device); the host looks at tr X - tr X^2 after each chunk
+        constexpr int CHUNK = 8;
+        for (int it = 0; it < 96 && !ok; it += CHUNK) {
+            for (int k = 0; k < CHUNK; ++k) {
+                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, Xp, n, Xp, n, 0.0, X2, n, S(ctx)));
+                HIPCHK(trace2(n, Xp, X2, (double)nocc, pur, S(ctx)));
+                HIPCHK(tc2_update((int64_t)n * n, pur + 2, Xp, X2, S(ctx)));
+            }
+            double h[3];
+            HIPCHK(hipMemcpyAsync(h, pur, 3 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            HIPCHK(hipStreamSynchronize(S(ctx)));
+            const double idem = h[0] - h[1];  // of the step before the last update
+            if (!std::isfinite(idem)) return FOCK_OK;
+            // quadratic convergence: below 1e-9 before the last step means below 1e-18 after it
+            if (idem < 1e-9) ok = std::fabs(h[0] - nocc) < 1e-8;
+        }
+        if (!ok) return FOCK_OK;
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, X_, n, Xp, n, 0.0, T1, n, S(ctx)));  // X Xp
+        HIPCHK(gemm_f64(false, true, n, n, n, 2.0, T1, n, X_, n, 0.0, P_, n, S(ctx)));  // 2 X Xp X^T
+        return FOCK_OK;
+    };
+    // vectors: the orbitals and orbital energies are needed (the returned iteration); otherwise
+    // the accurate policy takes the density from purification
+    auto diagonalize = [&](const double *M, bool vectors) -> fock_status {
         HIPCHK(gemm_f64(false, false, n, n, n, 1.0, M, n, X_, n, 0.0, T1, n, S(ctx)));  // M X
         HIPCHK(gemm_f64(true, false, n, n, n, 1.0, X_, n, T1, n, 0.0, T2, n, S(ctx)));  // X^T M X
         phase_mark("pre_eig");
+        if (!vectors && purify_ok) {
+            bool ok = false;
+            STCHK(purify(ok));
+            if (ok) {
+                phase_mark("purify");
+                return FOCK_OK;
+            }
+            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, M, n, X_, n, 0.0, T1, n, S(ctx)));
+            HIPCHK(gemm_f64(true, false, n, n, n, 1.0, X_, n, T1, n, 0.0, T2, n, S(ctx)));
+        }
         STCHK(eig_sym(ctx, n, T2, W_, dWork.as<double>(), dInfo.as<int>()));          // T2 = R'
         phase_mark("eig");
         HIPCHK(gemm_f64(false, true, n, n, n, 1.0, X_, n, T2, n, 0.0, C_, n, S(ctx)));  // C = X R'^T
@@ -1899,7 +1951,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     } else {
         // GWH guess (scf/src/lib.rs:150-162)
         HIPCHK(gwh_guess(n, S_, H_, F_, S(ctx)));
-        STCHK(diagonalize(F_));
+        STCHK(diagonalize(F_, false));
     }
 
     phase_mark("guess");
@@ -2043,8 +2095,10 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             Fuse = Fx_;
         }
         phase_mark("diis");
-        // F' = X^T F X, eigen, C = X C', P_new (:210-216)
-        STCHK(diagonalize(Fuse));
+        // F' = X^T F X, eigen, C = X C', P_new (:210-216); the orbitals are computed whenever this
+        // iteration may be the returned one
+        const bool may_exit = err < opts.epsilon || iter >= opts.max_iters - 1 || !is_normal(err);
+        STCHK(diagonalize(Fuse, may_exit));
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
         HIPCHK(energy_trace(n, P_, H_, G_, scal + 1, S(ctx)));
         double e_elec = 0.0;

@@ -239,6 +239,13 @@ hipError_t add_f64(int64_t len, const double *a, const double *b, double *out, h
 hipError_t transpose_f64(int n, const double *in, double *out, hipStream_t s);
 hipError_t fill_f64(int64_t len, double v, double *out, hipStream_t s);
 hipError_t copy_f64(int64_t len, const double *in, double *out, hipStream_t s);
+// trace-correcting purification helpers: Gershgorin bounds {lo, hi} of a symmetric matrix,
+// X = (hi I - A)/(hi - lo), out = {tr X, tr X2, grow} (grow: tr(2X - X2) closer to target than
+// tr X2), X = 2X - X2 if *grow else X2 (the choice stays on the device)
+hipError_t gershgorin_bounds(int n, const double *A, double *out, hipStream_t s);
+hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, hipStream_t s);
+hipError_t trace2(int n, const double *X, const double *X2, double target, double *out, hipStream_t s);
+hipError_t tc2_update(int64_t len, const double *grow, double *X, const double *X2, hipStream_t s);
 // GWH: out_ij = 0.875 S_ij (H_ii + H_jj)
 hipError_t gwh_guess(int n, const double *S, const double *H, double *out, hipStream_t s);
 // X = U diag(1/sqrt(D_ii)) U^T with D = U^T S U: scale columns of U by dinv (device vector)
