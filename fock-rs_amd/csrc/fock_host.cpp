@@ -1374,6 +1374,15 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     if (opts.diis_window < 1 || opts.diis_window > 16 || opts.diis_start < 1)
         return ctx->fail(FOCK_INVALID_ARG, "DIIS window must be 1..16 and start >= 1");
     const auto t_start = std::chrono::steady_clock::now();
+    const bool debug = std::getenv("FOCK_DEBUG") != nullptr;
+    auto t_mark = t_start;
+    auto phase_mark = [&](const char *what) {  // FOCK_DEBUG: wall time of each phase (synchronising)
+        if (!debug) return;
+        (void)hipStreamSynchronize(S(ctx));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[fock-batch] %-12s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(now - t_mark).count());
+        t_mark = now;
+    };
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = S(ctx);
     std::vector<std::unique_ptr<BatchMol>> mols;
@@ -1402,6 +1411,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         nmax = std::max(nmax, bm->n);
         mols.push_back(std::move(bm));
     }
+    phase_mark("prepare");
     if (!blocked_fits(ctx, nmax, false)) return ctx->fail(FOCK_UNSUPPORTED, "largest basis too large for a batch");
     // size groups: molecules of equal n share one batched eigensolver call
     std::map<int, std::vector<int>> groups;
@@ -1472,8 +1482,13 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             HIPCHK(gemm_f64(false, false, n, n, n, 1.0, fock_of(b), n, b.X.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
             HIPCHK(gemm_f64(true, false, n, n, n, 1.0, b.X.as<double>(), n, b.T1.as<double>(), n, 0.0, gA(b), n, st));
         }
-        for (auto &g : gr) STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
-                                           g->work.as<double>(), g->info.as<int>()));
+        for (auto &g : gr) {
+            bool any = false;  // a group whose molecules have all converged is skipped
+            for (int m : g->members) any = any || !mols[m]->done;
+            if (any)
+                STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
+                                g->work.as<double>(), g->info.as<int>()));
+        }
         for (auto &bp : mols) {
             BatchMol &b = *bp;
             if (b.done) continue;
@@ -1483,6 +1498,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         }
         return FOCK_OK;
     };
+    phase_mark("one_elec_X");
     // GWH guess (scf/src/lib.rs:150-162)
     for (auto &bp : mols) HIPCHK(gwh_guess(bp->n, bp->S.as<double>(), bp->H.as<double>(), bp->F.as<double>(), st));
     STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.F.as<double>(); }));
@@ -1549,13 +1565,17 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         return FOCK_OK;
     };
     STCHK(upload_records());
+    phase_mark("guess_tables");
     const bool refscreen = opts.screen == FOCK_SCREEN_REF_BF_PAIR;
     std::vector<double> hostd((size_t)count * 18);
     int active = count;
-    for (int iter = 0; iter <= opts.max_iters && active > 0; ++iter) {
-        // G(P), F = H + G for every active molecule: one launch per class for the whole batch
+    // G(P), F = H + G of the whole batch: one launch per quartet class; converged molecules'
+    // blocks return at once (their record is inactive). The sequence is the same every iteration
+    // (kernel arguments are device pointers), so it is captured once as a HIP graph and replayed;
+    // clears are fill kernels, not memsets. When capture is not possible (e.g. the legacy default
+    // stream), the launches are issued directly.
+    auto enqueue_build = [&]() -> fock_status {
         for (auto &bp : mols) {
-            if (bp->done) continue;
             const int64_t nn = (int64_t)bp->n * bp->n;
             HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Jb.as<double>(), st));
             HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Kb.as<double>(), st));
@@ -1573,13 +1593,40 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             HIPCHK(launch_eri_blocks(cls, a, opts.integral_policy, refscreen, false, st));
         }
         HIPCHK(scatter_jk_batch(count, d_scat.as<ScatterRec>(), max_npair, st));
+        for (auto &bp : mols)
+            HIPCHK(fock_from_jk(bp->n, bp->H.as<double>(), bp->pr.d_Jb.as<double>(), bp->pr.d_Kb.as<double>(),
+                                bp->G.as<double>(), bp->F.as<double>(), 0.5, st));
+        return FOCK_OK;
+    };
+    struct GraphGuard {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        ~GraphGuard() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } graph;
+    if (st != nullptr && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const fock_status cs = enqueue_build();
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (cs == FOCK_OK && ce == hipSuccess && g) {
+            graph.g = g;
+            if (hipGraphInstantiate(&graph.x, g, nullptr, nullptr, 0) != hipSuccess) graph.x = nullptr;
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();  // a failed capture leaves no launch behind; fall back to eager
+    }
+    for (int iter = 0; iter <= opts.max_iters && active > 0; ++iter) {
+        if (graph.x) HIPCHK(hipGraphLaunch(graph.x, st));
+        else STCHK(enqueue_build());
+        phase_mark("fock_build");
         for (auto &bp : mols) {
             BatchMol &b = *bp;
             if (b.done) continue;
             const int n = b.n;
             const int64_t nn = (int64_t)n * n;
-            HIPCHK(fock_from_jk(n, b.H.as<double>(), b.pr.d_Jb.as<double>(), b.pr.d_Kb.as<double>(), b.G.as<double>(),
-                                b.F.as<double>(), 0.5, st));
             // DIIS error FPS - SPF (scf/src/lib.rs:189-192) into the ring, dots with every slot
             const int slot = b.next_slot;
             double *e_slot = b.errs.as<double>() + (int64_t)slot * nn;
@@ -1636,7 +1683,9 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             }
             if (Fuse != b.Fx.as<double>()) HIPCHK(copy_f64(nn, Fuse, b.Fx.as<double>(), st));
         }
+        phase_mark("diis");
         STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.Fx.as<double>(); }));
+        phase_mark("diagonalize");
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (scf/src/lib.rs:223)
         for (int m = 0; m < count; ++m) {
             BatchMol &b = *mols[m];
@@ -1676,6 +1725,8 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             }
         }
         if (changed) STCHK(upload_records());
+        phase_mark("energy_exit");
+        if (debug) std::fprintf(stderr, "[fock-batch] iter %d active %d\n", iter, active);
     }
     HIPCHK(hipStreamSynchronize(st));
     const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

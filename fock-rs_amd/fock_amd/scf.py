@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Iterable, List, Optional
+from typing import Iterable, List, Optional, Sequence
 
 import numpy as np
 
@@ -122,6 +122,42 @@ def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_char
         coefficient_matrix=r["coefficients"], electronic_energy=r["electronic_energy"],
         total_energy=r["total_energy"], iterations=r["iterations"], n_electrons=r["n_electrons"],
         n_basis=r["n_basis"], history=r["history"], scf_wall_ms=r["scf_wall_ms"], converged=(st == abi.FOCK_OK))
+
+
+def _result(atoms, r, st) -> HartreeFockResult:
+    basis = [cg for a in atoms for cg in a.basis]
+    return HartreeFockResult(
+        orbitals=MolecularOrbitals(basis, r["coefficients"]),
+        orbital_energies=r["orbital_energies"], density_matrix=r["density"],
+        coefficient_matrix=r["coefficients"], electronic_energy=r["electronic_energy"],
+        total_energy=r["total_energy"], iterations=r["iterations"], n_electrons=r["n_electrons"],
+        n_basis=r["n_basis"], history=r.get("history", []), scf_wall_ms=r["scf_wall_ms"],
+        converged=(st == abi.FOCK_OK))
+
+
+def try_scf_batch(molecules: Sequence[Iterable[Atom]], max_iters: int, epsilon: float, molecule_charge: int = 0,
+                  policy: int = abi.POLICY_ACCURATE, screen: int = abi.SCREEN_SCHWARZ,
+                  schwarz_threshold: float = 1e-12, ctx: Optional[abi.Context] = None,
+                  **schedule) -> List[Optional[HartreeFockResult]]:
+    """try_scf for many molecules at once (batching form, FP64): the SCFs advance in lockstep with
+    one Fock-build launch per quartet class per iteration. None where an SCF diverged."""
+    mols = [list(a) for a in molecules]
+    flat = [flatten(a) for a in mols]
+    opts = abi.default_opts(max_iters=max_iters, epsilon=epsilon, molecule_charge=molecule_charge,
+                            integral_policy=policy, screen=screen, schwarz_threshold=schwarz_threshold, **schedule)
+    ctx = ctx or context()
+    _, statuses, results = ctx.scf_batch_run(flat, opts)
+    return [None if st == abi.FOCK_DIVERGED else _result(a, r, st) for a, r, st in zip(mols, results, statuses)]
+
+
+def nuclear_gradient(atoms: Iterable[Atom], result: HartreeFockResult, screen: int = abi.SCREEN_SCHWARZ,
+                     schwarz_threshold: float = 1e-12, ctx: Optional[abi.Context] = None) -> np.ndarray:
+    """dE/dR in Eh/Bohr, shape (n_atoms, 3), at an accurate-policy try_scf result."""
+    fb, fn = flatten(list(atoms))
+    opts = abi.default_opts(integral_policy=abi.POLICY_ACCURATE, screen=screen, schwarz_threshold=schwarz_threshold)
+    r = {"coefficients": result.coefficient_matrix, "density": result.density_matrix,
+         "orbital_energies": result.orbital_energies, "n_electrons": result.n_electrons}
+    return (ctx or context()).rhf_gradient(fb, fn, opts, r)
 
 
 class SelfConsistentField:

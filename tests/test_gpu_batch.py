@@ -56,3 +56,16 @@ def test_batch_rejects_mixed_precision(ctx):
     mols = clusters(2, seed=3)
     with pytest.raises(abi.FockError):
         ctx.scf_batch_run(mols, opts(precision=abi.PRECISION_MIXED))
+
+
+def test_python_facade_batch_and_gradient(ctx):
+    """try_scf_batch / nuclear_gradient: the scf-facade forms of the batched SCF and the gradient."""
+    from fock_amd.scf import nuclear_gradient, try_scf, try_scf_batch
+    mols = [m.build(g, "6-31G") for g in m.water_cluster_batch(3, seed=5)]
+    sched = dict(ROBUST)
+    res = try_scf_batch(mols, 200, 1e-10, ctx=ctx, **sched)
+    for atoms, r in zip(mols, res):
+        one = try_scf(atoms, 200, 1e-10, 0, policy=abi.POLICY_ACCURATE, screen=abi.SCREEN_SCHWARZ, ctx=ctx, **sched)
+        assert r.converged and abs(r.total_energy - one.total_energy) < 1e-9
+        g = nuclear_gradient(atoms, r, ctx=ctx)
+        assert g.shape == (len(atoms), 3) and np.abs(g.sum(axis=0)).max() < 1e-10
