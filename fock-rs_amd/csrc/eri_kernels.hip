@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void prim_count_kernel(PrimCountArgs args) {
 // contribution of such a task lands in row a or row b (K[a][c], K[a][d], K[b][c], K[b][d]), so the
 // workgroup accumulates those rows in LDS with LDS atomics and flushes each nonzero entry once;
 // J[a][b] is reduced over the workgroup; only J[c][d] is a per-lane global atomic.
-template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool BATCH>
 __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     extern __shared__ __align__(32) unsigned char lds_bytes[];
     JKArgs args = args_in;
     Block blk_;
-    if (args_in.batch) {
+    if constexpr (BATCH) {
         // batched launch: this block belongs to one molecule's record
         const int2 bb = args_in.bblocks[blockIdx.x];
         const BatchRec r = args_in.batch[bb.x];
@@ -655,7 +655,7 @@ size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32) {
            (policy == 1 ? BOYS_GRID_N * cols * real : 0);
 }
 
-template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool BATCH = false>
 static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
     constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
@@ -665,17 +665,24 @@ static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
                                       sizeof(unsigned long long)) +
                        MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) +
                        (POLICY == 1 ? (size_t)BOYS_GRID_N * COLS * sizeof(Real) : 0);
-    hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real>), dim3(a.nblocks), dim3(256), lds, s,
-                       a);
+    hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real, BATCH>), dim3(a.nblocks), dim3(256),
+                       lds, s, a);
     return hipGetLastError();
 }
 
 template <int LA, int LB, int LC, int LD>
 static hipError_t dispatch_block_policy(const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s) {
     if (fp32) {
-        if (policy != 1) return hipErrorInvalidValue;
+        if (policy != 1 || a.batch) return hipErrorInvalidValue;  // batches run the FP64 form
         return refscreen ? launch_block<LA, LB, LC, LD, 1, true, float>(a, s)
                          : launch_block<LA, LB, LC, LD, 1, false, float>(a, s);
+    }
+    if (a.batch) {  // a separate instantiation keeps the one-molecule kernels' registers as they are
+        if (policy == 0)
+            return refscreen ? launch_block<LA, LB, LC, LD, 0, true, double, true>(a, s)
+                             : launch_block<LA, LB, LC, LD, 0, false, double, true>(a, s);
+        return refscreen ? launch_block<LA, LB, LC, LD, 1, true, double, true>(a, s)
+                         : launch_block<LA, LB, LC, LD, 1, false, double, true>(a, s);
     }
     if (policy == 0)
         return refscreen ? launch_block<LA, LB, LC, LD, 0, true, double>(a, s)
