@@ -42,6 +42,9 @@ constexpr double F_PRIM[NUM_CLASSES] = {38, 85, 266, 298, 977, 3833};
 // primitive screen of the accurate policy: a primitive quartet whose Cauchy-Schwarz bound
 // q_ij q_kl (contraction coefficients included) is at most this is not evaluated
 constexpr float PRIM_THRESHOLD = 1e-15f;
+// mixed-precision gradient: derivative integrals of tasks whose Schwarz bound Q_ab Q_cd (largest
+// ket of the task) is below this run in FP32, the others in FP64
+constexpr float GRAD_TAU64 = 1e-4f;
 constexpr int CLASS_NINT[NUM_CLASSES] = {1, 3, 9, 9, 27, 81};
 
 struct DevBuf {
@@ -437,6 +440,15 @@ namespace {
 // Shell pairs, primitive-pair data, screen lists, tasks and blocks of one basis into pr.
 fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *basis, int32_t policy, int32_t screen,
                          double tau, int32_t precision) {
+    const bool debug = std::getenv("FOCK_DEBUG") != nullptr;
+    auto t_mark = std::chrono::steady_clock::now();
+    auto mark = [&](const char *what) {  // FOCK_DEBUG: wall time of each preparation step
+        if (!debug) return;
+        (void)hipStreamSynchronize(S(ctx));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[fock-prep] %-14s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(now - t_mark).count());
+        t_mark = now;
+    };
     pr.valid = false;
     STCHK(build_shells(ctx, basis, pr.shells));
     const int n = basis->n_functions;
@@ -474,6 +486,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
                 }
             pairs.push_back(ph);
         }
+    mark("pairs_prims");
     const int npair = (int)pairs.size();
     const size_t nprim = prims.size();
     HIPCHK(pr.d_prims64.ensure(nprim * sizeof(PrimPair<double>)));
@@ -541,6 +554,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         HIPCHK(hipStreamSynchronize(S(ctx)));
     }
 
+    mark("prim_screen");
     HIPCHK(pr.d_prim_ab.ensure(std::max<size_t>(nprim, 1) * sizeof(double2)));
     HIPCHK(hipMemcpyAsync(pr.d_prim_ab.p, prim_ab.data(), nprim * sizeof(double2), hipMemcpyHostToDevice, S(ctx)));
     HIPCHK(hipStreamSynchronize(S(ctx)));  // prim_ab is a host temporary
@@ -580,6 +594,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         ph.dmax = std::isnan(m) ? HUGE_VAL : m;  // NaN inputs: keep the pair, sort stays well defined
     }
 
+    mark("diag_D");
     // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation)
     std::vector<int> order(npair);
     std::iota(order.begin(), order.end(), 0);
@@ -624,6 +639,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         buckets.push_back(Bucket{pr.pairs[i].cls, pr.pairs[i].K, i, j});
         i = j;
     }
+    mark("sort_buckets");
     {
         // ket-side primitive data, structure of arrays per bucket: field f of primitive k of the
         // bucket's pair j at base + (k * PRIM_FIELDS + f) * nb + (j - begin), so the 64 lanes of a
@@ -671,6 +687,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         if (screen == FOCK_SCREEN_SCHWARZ) return std::sqrt(di) * std::sqrt(dj) > tau;
         return di * dj > pr.dthr;
     };
+    mark("soa");
     std::vector<Task> tasks;
     std::vector<Block> blocks_all;
     std::vector<int> gfirst_all, glast_all;
@@ -756,6 +773,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     }
     pr.task_off[NUM_CLASSES] = (int)tasks.size();
     pr.block_off[NUM_CLASSES] = (int)blocks_all.size();
+    mark("tasks_blocks");
     HIPCHK(pr.d_blocks.ensure(std::max<size_t>(blocks_all.size(), 1) * sizeof(Block)));
     HIPCHK(hipMemcpyAsync(pr.d_blocks.p, blocks_all.data(), blocks_all.size() * sizeof(Block), hipMemcpyHostToDevice,
                           S(ctx)));
@@ -813,6 +831,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     HIPCHK(pr.d_shell_nf.ensure(ns * sizeof(int)));
     HIPCHK(hipMemcpyAsync(pr.d_shell_f0.p, f0.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
     HIPCHK(hipMemcpyAsync(pr.d_shell_nf.p, nf.data(), ns * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+    mark("uploads");
     pr.prim_counted = !pr.prim_screen;  // kept primitive quartets: counted on the first stats request
     HIPCHK(hipStreamSynchronize(S(ctx)));
     pr.valid = true;
@@ -1758,7 +1777,8 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
     if (nocc < 0 || nocc > n) return ctx->fail(FOCK_INVALID_ARG, "bad electron count");
     if ((size_t)3 * na * sizeof(double) > ctx->max_lds) return ctx->fail(FOCK_UNSUPPORTED, "too many atoms");
     HIPCHK(hipSetDevice(ctx->device));
-    STCHK(fock_jk_prepare(ctx, basis, FOCK_POLICY_ACCURATE, opts.screen, opts.schwarz_threshold, 0));
+    const bool mixed = opts.precision == FOCK_PRECISION_MIXED;
+    STCHK(fock_jk_prepare(ctx, basis, FOCK_POLICY_ACCURATE, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0));
     Prepared &pr = ctx->jk;
     const int ns = (int)pr.shells.size();
     // the nucleus each shell sits on (derivatives move basis functions with their nucleus)
@@ -1816,6 +1836,11 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
         g.pairs = pr.d_pairs.as<PairInfo>();
         g.prims = pr.d_prims64.as<PrimPair<double>>();
         g.prim_ab = pr.d_prim_ab.as<double2>();
+        if (mixed) {
+            g.prims32 = pr.d_prims32.as<PrimPair<float>>();
+            g.pairQ = pr.d_pairQ.as<float>();
+            g.tau64 = GRAD_TAU64;
+        }
         g.pair_shell = pr.d_pair_shell.as<int2>();
         g.shell_atom = dAtom.as<int>();
         g.P = dP.as<double>();

@@ -20,42 +20,45 @@ namespace fock {
 
 // One primitive quartet: the nine derivatives (A, B, C; x, y, z) of every component integral,
 // contracted with fac[ab][cd] (the pair-density weight of each component) into g[9].
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &bra, const PrimPair<double> &ket,
-                                                        double alpha_a, double alpha_b, double alpha_c,
+template <int LA, int LB, int LC, int LD, typename Real>
+__device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<Real> &bra, const PrimPair<Real> &ket,
+                                                        Real alpha_a, Real alpha_b, Real alpha_c,
                                                         const BoysTables &bt,
                                                         const double fac[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)],
                                                         double g[9]) {
     constexpr int L1 = LA + LB + LC + LD + 1;
     constexpr int LAB = LA + LB, LCD = LC + LD;
     constexpr int NB = ncart(LB), ND = ncart(LD), NC = ncart(LC), NA = ncart(LA);
-    const double p = bra.p, q = ket.p;
-    const double rs = rsqrt(p + q);
-    const double alpha = p * q * (rs * rs);
-    const double X = ket.Px - bra.Px, Y = ket.Py - bra.Py, Z = ket.Pz - bra.Pz;  // Q - P
-    const double T = alpha * (X * X + Y * Y + Z * Z);
-    const double pref = 34.98683665524972 * bra.Kc * ket.Kc * rs;  // 2 pi^{5/2}/(p q sqrt(p+q)) K K
-    double Fs[L1 + 1];
-    boys_taylor64<L1>(T, bt.tab64, Fs);
+    const Real p = bra.p, q = ket.p;
+    Real rs;
+    if constexpr (sizeof(Real) == 8) rs = rsqrt(p + q);
+    else rs = rsqrtf(p + q);
+    const Real alpha = p * q * (rs * rs);
+    const Real X = ket.Px - bra.Px, Y = ket.Py - bra.Py, Z = ket.Pz - bra.Pz;  // Q - P
+    const Real T = alpha * (X * X + Y * Y + Z * Z);
+    const Real pref = Real(34.98683665524972) * bra.Kc * ket.Kc * rs;  // 2 pi^{5/2}/(p q sqrt(p+q)) K K
+    Real Fs[L1 + 1];
+    if constexpr (sizeof(Real) == 8) boys_taylor64<L1>(T, bt.tab64, Fs);
+    else boys_taylor32<L1>(T, bt.tab32, Fs);
     {
-        const double m2a = -2.0 * alpha;
-        double f = m2a;
+        const Real m2a = Real(-2) * alpha;
+        Real f = m2a;
 #pragma unroll
         for (int n = 1; n <= L1; ++n) {
             Fs[n] *= f;
             f *= m2a;
         }
     }
-    double R[L1 + 1][L1 + 1][L1 + 1];
-    hermite_coulomb<L1, false, double>(Fs, X, Y, Z, R);
-    double Eab[3][LA + 2][LB + 2][LAB + 3];
-    double Ecd[3][LC + 2][LD + 1][LCD + 2];
-    hermite_1d<LA + 1, LB + 1, double>(bra.xax, bra.xbx, bra.inv2p, Eab[0]);
-    hermite_1d<LA + 1, LB + 1, double>(bra.xay, bra.xby, bra.inv2p, Eab[1]);
-    hermite_1d<LA + 1, LB + 1, double>(bra.xaz, bra.xbz, bra.inv2p, Eab[2]);
-    hermite_1d<LC + 1, LD, double>(ket.xax, ket.xbx, ket.inv2p, Ecd[0]);
-    hermite_1d<LC + 1, LD, double>(ket.xay, ket.xby, ket.inv2p, Ecd[1]);
-    hermite_1d<LC + 1, LD, double>(ket.xaz, ket.xbz, ket.inv2p, Ecd[2]);
+    Real R[L1 + 1][L1 + 1][L1 + 1];
+    hermite_coulomb<L1, false, Real>(Fs, X, Y, Z, R);
+    Real Eab[3][LA + 2][LB + 2][LAB + 3];
+    Real Ecd[3][LC + 2][LD + 1][LCD + 2];
+    hermite_1d<LA + 1, LB + 1, Real>(bra.xax, bra.xbx, bra.inv2p, Eab[0]);
+    hermite_1d<LA + 1, LB + 1, Real>(bra.xay, bra.xby, bra.inv2p, Eab[1]);
+    hermite_1d<LA + 1, LB + 1, Real>(bra.xaz, bra.xbz, bra.inv2p, Eab[2]);
+    hermite_1d<LC + 1, LD, Real>(ket.xax, ket.xbx, ket.inv2p, Ecd[0]);
+    hermite_1d<LC + 1, LD, Real>(ket.xay, ket.xby, ket.inv2p, Ecd[1]);
+    hermite_1d<LC + 1, LD, Real>(ket.xaz, ket.xbz, ket.inv2p, Ecd[2]);
 
     static_for<0, NC>([&](auto c_) {
         constexpr int c = decltype(c_)::value;
@@ -65,22 +68,22 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
             constexpr int dx = cart(LD, d, 0), dy = cart(LD, d, 1), dz = cart(LD, d, 2);
             // ket contractions over the bra Hermite simplex: W0 with the ket as it is (bra Hermite
             // order up to LAB + 1 for the shifted bra), WC[axis] with the ket's C derivative
-            double W0[LAB + 2][LAB + 2][LAB + 2];
-            double WC[3][LAB + 1][LAB + 1][LAB + 1];
+            Real W0[LAB + 2][LAB + 2][LAB + 2];
+            Real WC[3][LAB + 1][LAB + 1][LAB + 1];
             static_for<0, LAB + 2>([&](auto t_) {
                 constexpr int t = decltype(t_)::value;
                 static_for<0, LAB + 2 - t>([&](auto u_) {
                     constexpr int u = decltype(u_)::value;
                     static_for<0, LAB + 2 - t - u>([&](auto v_) {
                         constexpr int v = decltype(v_)::value;
-                        double acc = 0.0;
+                        Real acc = Real(0);
                         static_for<0, cx + dx + 1>([&](auto tt_) {
                             constexpr int tt = decltype(tt_)::value;
                             static_for<0, cy + dy + 1>([&](auto uu_) {
                                 constexpr int uu = decltype(uu_)::value;
                                 static_for<0, cz + dz + 1>([&](auto vv_) {
                                     constexpr int vv = decltype(vv_)::value;
-                                    const double e = Ecd[0][cx][dx][tt] * Ecd[1][cy][dy][uu] * Ecd[2][cz][dz][vv];
+                                    const Real e = Ecd[0][cx][dx][tt] * Ecd[1][cy][dy][uu] * Ecd[2][cz][dz][vv];
                                     if constexpr ((tt + uu + vv) & 1) acc -= e * R[t + tt][u + uu][v + vv];
                                     else acc += e * R[t + tt][u + uu][v + vv];
                                 });
@@ -94,29 +97,29 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
                                 constexpr int ck = k == 0 ? cx : (k == 1 ? cy : cz);
                                 constexpr int dk = k == 0 ? dx : (k == 1 ? dy : dz);
                                 constexpr int nx = cx + dx + (k == 0), ny = cy + dy + (k == 1), nz = cz + dz + (k == 2);
-                                double accc = 0.0;
+                                Real accc = Real(0);
                                 static_for<0, nx + 1>([&](auto tt_) {
                                     constexpr int tt = decltype(tt_)::value;
                                     static_for<0, ny + 1>([&](auto uu_) {
                                         constexpr int uu = decltype(uu_)::value;
                                         static_for<0, nz + 1>([&](auto vv_) {
                                             constexpr int vv = decltype(vv_)::value;
-                                            double ek = 0.0;
+                                            Real ek = Real(0);
                                             constexpr int h = k == 0 ? tt : (k == 1 ? uu : vv);
-                                            if constexpr (h <= ck + 1 + dk) ek = -2.0 * alpha_c * Ecd[k][ck + 1][dk][h];
-                                            if constexpr (ck >= 1 && h <= ck - 1 + dk) ek += (double)ck * Ecd[k][ck - 1][dk][h];
-                                            double e = ek;
+                                            if constexpr (h <= ck + 1 + dk) ek = Real(-2) * alpha_c * Ecd[k][ck + 1][dk][h];
+                                            if constexpr (ck >= 1 && h <= ck - 1 + dk) ek += Real(ck) * Ecd[k][ck - 1][dk][h];
+                                            Real e = ek;
                                             if constexpr (k != 0) {
                                                 if constexpr (tt <= cx + dx) e *= Ecd[0][cx][dx][tt];
-                                                else e = 0.0;
+                                                else e = Real(0);
                                             }
                                             if constexpr (k != 1) {
                                                 if constexpr (uu <= cy + dy) e *= Ecd[1][cy][dy][uu];
-                                                else e = 0.0;
+                                                else e = Real(0);
                                             }
                                             if constexpr (k != 2) {
                                                 if constexpr (vv <= cz + dz) e *= Ecd[2][cz][dz][vv];
-                                                else e = 0.0;
+                                                else e = Real(0);
                                             }
                                             if constexpr ((tt + uu + vv) & 1) accc -= e * R[t + tt][u + uu][v + vv];
                                             else accc += e * R[t + tt][u + uu][v + vv];
@@ -135,15 +138,15 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
                 static_for<0, NB>([&](auto b_) {
                     constexpr int b = decltype(b_)::value;
                     constexpr int bx = cart(LB, b, 0), by = cart(LB, b, 1), bz = cart(LB, b, 2);
-                    const double f = pref * fac[a * NB + b][c * ND + d];
-                    double dA[3] = {0.0, 0.0, 0.0}, dB[3] = {0.0, 0.0, 0.0}, dC[3] = {0.0, 0.0, 0.0};
+                    const double f = (double)pref * fac[a * NB + b][c * ND + d];
+                    Real dA[3] = {Real(0), Real(0), Real(0)}, dB[3] = {Real(0), Real(0), Real(0)}, dC[3] = {Real(0), Real(0), Real(0)};
                     // d/dA, d/dB along axis k: the bra E of that axis shifted on a or b
                     static_for<0, 3>([&](auto k_) {
                         constexpr int k = decltype(k_)::value;
                         constexpr int ak = k == 0 ? ax : (k == 1 ? ay : az);
                         constexpr int bk = k == 0 ? bx : (k == 1 ? by : bz);
                         constexpr int mx = ax + bx + (k == 0), my = ay + by + (k == 1), mz = az + bz + (k == 2);
-                        double sa = 0.0, sb = 0.0;
+                        Real sa = Real(0), sb = Real(0);
                         static_for<0, mx + 1>([&](auto t_) {
                             constexpr int t = decltype(t_)::value;
                             static_for<0, my + 1>([&](auto u_) {
@@ -151,22 +154,22 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
                                 static_for<0, mz + 1>([&](auto v_) {
                                     constexpr int v = decltype(v_)::value;
                                     constexpr int h = k == 0 ? t : (k == 1 ? u : v);
-                                    double ea = -2.0 * alpha_a * Eab[k][ak + 1][bk][h];
-                                    double eb = -2.0 * alpha_b * Eab[k][ak][bk + 1][h];
-                                    if constexpr (ak >= 1 && h <= ak - 1 + bk) ea += (double)ak * Eab[k][ak - 1][bk][h];
-                                    if constexpr (bk >= 1 && h <= ak + bk - 1) eb += (double)bk * Eab[k][ak][bk - 1][h];
-                                    double rest = W0[t][u][v];
+                                    Real ea = Real(-2) * alpha_a * Eab[k][ak + 1][bk][h];
+                                    Real eb = Real(-2) * alpha_b * Eab[k][ak][bk + 1][h];
+                                    if constexpr (ak >= 1 && h <= ak - 1 + bk) ea += Real(ak) * Eab[k][ak - 1][bk][h];
+                                    if constexpr (bk >= 1 && h <= ak + bk - 1) eb += Real(bk) * Eab[k][ak][bk - 1][h];
+                                    Real rest = W0[t][u][v];
                                     if constexpr (k != 0) {
                                         if constexpr (t <= ax + bx) rest *= Eab[0][ax][bx][t];
-                                        else rest = 0.0;
+                                        else rest = Real(0);
                                     }
                                     if constexpr (k != 1) {
                                         if constexpr (u <= ay + by) rest *= Eab[1][ay][by][u];
-                                        else rest = 0.0;
+                                        else rest = Real(0);
                                     }
                                     if constexpr (k != 2) {
                                         if constexpr (v <= az + bz) rest *= Eab[2][az][bz][v];
-                                        else rest = 0.0;
+                                        else rest = Real(0);
                                     }
                                     sa += ea * rest;
                                     sb += eb * rest;
@@ -175,7 +178,7 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
                         });
                         dA[k] = sa;
                         dB[k] = sb;
-                        double sc = 0.0;
+                        Real sc = Real(0);
                         static_for<0, ax + bx + 1>([&](auto t_) {
                             constexpr int t = decltype(t_)::value;
                             static_for<0, ay + by + 1>([&](auto u_) {
@@ -190,9 +193,9 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
                     });
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        g[k] += f * dA[k];
-                        g[3 + k] += f * dB[k];
-                        g[6 + k] += f * dC[k];
+                        g[k] += f * (double)dA[k];
+                        g[3 + k] += f * (double)dB[k];
+                        g[6 + k] += f * (double)dC[k];
                     }
                 });
             });
@@ -201,7 +204,7 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<double> &
 }
 
 // Two-electron part: one lane per unique shell quartet of the Fock build's task list.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real>
 __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
@@ -210,8 +213,17 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int t = blockIdx.x * 4 + wave;
-    if (t < args.ntasks) {
-        const Task task = args.tasks[t];
+    bool run = t < args.ntasks;
+    Task task{};
+    if (run) {
+        task = args.tasks[t];
+        if (args.pairQ) {
+            // mixed form: the task's largest bound (its first ket) picks the precision pass
+            const float bound = args.pairQ[task.bra] * args.pairQ[task.ket_begin];
+            run = (sizeof(Real) == 8) == (bound >= args.tau64);
+        }
+    }
+    if (run) {
         const PairInfo bra = args.pairs[task.bra];
         const int2 sab = args.pair_shell[task.bra];
         const int atA = args.shell_atom[sab.x], atB = args.shell_atom[sab.y];
@@ -243,13 +255,16 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
                                                0.25 * (P[ia * N + ic] * P[ib * N + id] + P[ia * N + id] * P[ib * N + ic]);
                             fac[a * NB + b][c * ND + d] = w * gam;
                         }
+            const PrimPair<Real> *__restrict__ prims = reinterpret_cast<const PrimPair<Real> *>(
+                sizeof(Real) == 8 ? (const void *)args.prims : (const void *)args.prims32);
             for (int i = 0; i < bra.K; ++i) {
-                const PrimPair<double> bp = args.prims[bra.pp_start + i];
+                const PrimPair<Real> bp = prims[bra.pp_start + i];
                 const double2 ab = args.prim_ab[bra.pp_start + i];
                 for (int k = 0; k < ket.K; ++k) {
-                    const PrimPair<double> kp = args.prims[ket.pp_start + k];
+                    const PrimPair<Real> kp = prims[ket.pp_start + k];
                     const double2 cd = args.prim_ab[ket.pp_start + k];
-                    accumulate_grad_quartet<LA, LB, LC, LD>(bp, kp, ab.x, ab.y, cd.x, args.bt, fac, g);
+                    accumulate_grad_quartet<LA, LB, LC, LD, Real>(bp, kp, (Real)ab.x, (Real)ab.y, (Real)cd.x, args.bt,
+                                                                  fac, g);
                 }
             }
         }
@@ -443,20 +458,27 @@ hipError_t energy_weighted_density(int n, int nocc, const double *C, const doubl
     return hipGetLastError();
 }
 
-hipError_t launch_grad_eri(int cls, const GradEriArgs &a, hipStream_t s) {
-    if (a.ntasks <= 0) return hipSuccess;
+template <typename Real>
+static hipError_t launch_grad_eri_prec(int cls, const GradEriArgs &a, hipStream_t s) {
     const dim3 grid((a.ntasks + 3) / 4), block(256);
     const size_t lds = (size_t)3 * a.natoms * sizeof(double);
     switch (cls) {
-        case 0: hipLaunchKernelGGL((grad_eri_kernel<0, 0, 0, 0>), grid, block, lds, s, a); break;
-        case 1: hipLaunchKernelGGL((grad_eri_kernel<1, 0, 0, 0>), grid, block, lds, s, a); break;
-        case 2: hipLaunchKernelGGL((grad_eri_kernel<1, 0, 1, 0>), grid, block, lds, s, a); break;
-        case 3: hipLaunchKernelGGL((grad_eri_kernel<1, 1, 0, 0>), grid, block, lds, s, a); break;
-        case 4: hipLaunchKernelGGL((grad_eri_kernel<1, 1, 1, 0>), grid, block, lds, s, a); break;
-        case 5: hipLaunchKernelGGL((grad_eri_kernel<1, 1, 1, 1>), grid, block, lds, s, a); break;
+        case 0: hipLaunchKernelGGL((grad_eri_kernel<0, 0, 0, 0, Real>), grid, block, lds, s, a); break;
+        case 1: hipLaunchKernelGGL((grad_eri_kernel<1, 0, 0, 0, Real>), grid, block, lds, s, a); break;
+        case 2: hipLaunchKernelGGL((grad_eri_kernel<1, 0, 1, 0, Real>), grid, block, lds, s, a); break;
+        case 3: hipLaunchKernelGGL((grad_eri_kernel<1, 1, 0, 0, Real>), grid, block, lds, s, a); break;
+        case 4: hipLaunchKernelGGL((grad_eri_kernel<1, 1, 1, 0, Real>), grid, block, lds, s, a); break;
+        case 5: hipLaunchKernelGGL((grad_eri_kernel<1, 1, 1, 1, Real>), grid, block, lds, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_grad_eri(int cls, const GradEriArgs &a, hipStream_t s) {
+    if (a.ntasks <= 0) return hipSuccess;
+    hipError_t e = launch_grad_eri_prec<double>(cls, a, s);
+    if (e == hipSuccess && a.pairQ) e = launch_grad_eri_prec<float>(cls, a, s);  // mixed: FP32 pass
+    return e;
 }
 
 hipError_t launch_grad_one(int pair_class, const GradOneArgs &a, hipStream_t s) {

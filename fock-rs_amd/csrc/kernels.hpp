@@ -159,7 +159,12 @@ struct GradEriArgs {
     int ntasks;
     const PairInfo *pairs;
     const PrimPair<double> *prims;
+    const PrimPair<float> *prims32;  // mixed form
     const double2 *prim_ab;  // exponents (alpha_a, alpha_b) of every primitive pair
+    // mixed form (pairQ not null): tasks whose bound Q_bra Q_ket(first) reaches tau64 run in FP64,
+    // the others in FP32; null: every task in FP64
+    const float *pairQ;
+    float tau64;
     const int2 *pair_shell;
     const int *shell_atom;
     const double *P;

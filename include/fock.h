@@ -186,7 +186,9 @@ fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fock_basis_de
  * 1/2 sum (P P - exchange) d(ab|cd) + dE_nuc, W = 2 sum_occ eps_i C_i C_i^T, with the energy of
  * scf/src/lib.rs:223 and the nuclear repulsion of :102-116. Accurate integral policy only (the
  * reference Boys table is not differentiable); opts->screen / schwarz_threshold select the
- * quartets as in the SCF. Every basis-function centre must coincide with a nucleus. */
+ * quartets as in the SCF; opts->precision = FOCK_PRECISION_MIXED evaluates the derivative
+ * integrals of quartets with Schwarz bound below 1e-4 in FP32 (FP64 accumulation). Every
+ * basis-function centre must coincide with a nucleus. */
 fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
                               const fock_scf_opts *opts, const fock_scf_out *scf, double *gradient);
 

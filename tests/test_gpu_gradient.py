@@ -108,3 +108,23 @@ def test_autograd_function_matches_fd(ctx):
     want = refscf.fd_gradient(fb, fn)
     assert abs(float(e) - refscf.try_scf(fb, fn, 400, 1e-13, 0, accurate=True).total_energy) < 1e-10
     assert np.abs(pos.grad.numpy() - want).max() < TOL
+
+
+@pytest.mark.parametrize("which", ["water4", "octane"])
+def test_gradient_mixed_precision_matches_fp64(ctx, which):
+    """Mixed form: derivative integrals of small-bound quartets in FP32 (FP64 above a Schwarz bound
+    of 1e-4), the same converged density; within 1e-6 Eh/Bohr of the FP64 gradient."""
+    if which == "water4":
+        fb, fn = m.flatten(m.build(m.water_cluster_geometry(4, np.random.default_rng(4)), "6-31G"))
+        o = opts(screen=abi.SCREEN_SCHWARZ, tau=1e-12, eps=1e-10)
+    else:
+        fb, fn = m.flatten(m.octane_631g())
+        o = opts(screen=abi.SCREEN_SCHWARZ, tau=1e-12, eps=1e-10, **ROBUST)
+    r = energy(ctx, fb, fn, o)
+    g64 = ctx.rhf_gradient(fb, fn, o, r)
+    om = opts(screen=abi.SCREEN_SCHWARZ, tau=1e-12, eps=1e-10, precision=abi.PRECISION_MIXED)
+    g32 = ctx.rhf_gradient(fb, fn, om, r)
+    err = np.abs(g32 - g64).max()
+    print(which, "max |mixed - fp64| = %.2e" % err)
+    assert err < TOL
+    assert np.abs(g32.sum(axis=0)).max() < 1e-9
