@@ -400,8 +400,10 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     JKArgs args = args_in;
     Block blk_;
     if constexpr (BATCH) {
-        // batched launch: this block belongs to one molecule's record
+        // batched launch: this block belongs to one molecule's record (record -1: a padding slot
+        // of an XCD-affine schedule, no work)
         const int2 bb = args_in.bblocks[blockIdx.x];
+        if (bb.x < 0) return;  // the same for every thread of the block
         const BatchRec r = args_in.batch[bb.x];
         if (!r.active) return;  // the same for every thread of the block
         args.tasks = r.tasks;
@@ -431,11 +433,12 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
     double *red = Kb + NB * n;        // [4][NAB]
-    Task *s_task = reinterpret_cast<Task *>(red + 4 * NAB);              // [BLOCK_TASKS]
-    PairInfo *s_ket = reinterpret_cast<PairInfo *>(s_task + BLOCK_TASKS);  // [BLOCK_TASKS * 64] kept kets (SoA starts)
-    int2 *s_kx = reinterpret_cast<int2 *>(s_ket + BLOCK_TASKS * 64);       // [BLOCK_TASKS * 64] (pair index, bucket size)
-    int *s_stride = reinterpret_cast<int *>(s_kx + BLOCK_TASKS * 64);      // [BLOCK_TASKS] bucket sizes
-    unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_stride + 2 * BLOCK_TASKS);  // [BLOCK_TASKS]
+    const int tile = args_in.tile;  // most tasks per block of this launch (blk.ntasks <= tile)
+    Task *s_task = reinterpret_cast<Task *>(red + 4 * NAB);         // [tile]
+    PairInfo *s_ket = reinterpret_cast<PairInfo *>(s_task + tile);  // [tile * 64] kept kets (SoA starts)
+    int2 *s_kx = reinterpret_cast<int2 *>(s_ket + tile * 64);       // [tile * 64] (pair index, bucket size)
+    int *s_stride = reinterpret_cast<int *>(s_kx + tile * 64);      // [tile] bucket sizes (2 tile ints: alignment)
+    unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_stride + 2 * tile);  // [tile]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
@@ -644,13 +647,15 @@ static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool ref
                      : launch_one<LA, LB, LC, LD, 1, MODE, false, double>(a, s);
 }
 
-size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32) {
+// LDS of one workgroup of the blocked digestion: K rows a and b, the J_ab reduction, the tile's
+// task and kept-ket records, the bra primitive pairs with their q, and the class's Boys slab
+size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile) {
     const int na = CLASS_BRA[cls] == 0 ? 1 : 3, nb = CLASS_BRA[cls] == 2 ? 3 : 1;
     const size_t cols = (fp32 ? BOYS_TAYLOR32 : BOYS_TAYLOR64) + 1, real = fp32 ? sizeof(float) : sizeof(double);
     const size_t pp = fp32 ? sizeof(PrimPair<float>) : sizeof(PrimPair<double>);
     return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
-           BLOCK_TASKS * (sizeof(Task) + 64 * (sizeof(PairInfo) + sizeof(int2)) + 2 * sizeof(int) +
-                          sizeof(unsigned long long)) +
+           (size_t)tile * (sizeof(Task) + 64 * (sizeof(PairInfo) + sizeof(int2)) + 2 * sizeof(int) +
+                           sizeof(unsigned long long)) +
            MAX_PRIM_PAIRS * (pp + sizeof(float)) +
            (policy == 1 ? BOYS_GRID_N * cols * real : 0);
 }
@@ -658,13 +663,9 @@ size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32) {
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool BATCH = false>
 static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
-    constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
-    const size_t lds = (size_t)(ncart(LA) + ncart(LB)) * a.nbf * sizeof(double) +
-                       4 * ncart(LA) * ncart(LB) * sizeof(double) +
-                       BLOCK_TASKS * (sizeof(Task) + 64 * (sizeof(PairInfo) + sizeof(int2)) + 2 * sizeof(int) +
-                                      sizeof(unsigned long long)) +
-                       MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) +
-                       (POLICY == 1 ? (size_t)BOYS_GRID_N * COLS * sizeof(Real) : 0);
+    if (a.tile < 1 || a.tile > MAX_TILE) return hipErrorInvalidValue;
+    constexpr int cls = (LA + LB == 0) ? 0 : (LA + LB == 1) ? (LC + LD == 0 ? 1 : 2) : (LC + LD == 0 ? 3 : LC + LD == 1 ? 4 : 5);
+    const size_t lds = block_lds_bytes(cls, a.nbf, POLICY, sizeof(Real) == 4, a.tile);
     hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real, BATCH>), dim3(a.nblocks), dim3(256),
                        lds, s, a);
     return hipGetLastError();

@@ -96,6 +96,9 @@ struct Prepared {
     bool prim_counted = false;  // stats.primitive_quartets holds the kept (screened) count
     int task_off[NUM_CLASSES + 1] = {0};
     int block_off[NUM_CLASSES + 1] = {0};
+    std::vector<Task> h_tasks;    // the task lists of d_tasks (block lists are rebuilt from them)
+    std::vector<Block> h_blocks;  // the block lists of d_blocks
+    int tile[NUM_CLASSES] = {DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE};
     fock_jk_stats stats{};
     DevBuf d_Jb, d_Kb, d_P, d_J, d_K;
     // shell metadata on device (density-weighted screening)
@@ -123,6 +126,11 @@ struct fock_ctx {
     DevBuf lut, tab64, tab32, pd64, pd32;
     BoysTables bt{};
     Prepared jk;
+    // tasks per block of the blocked digestion per quartet class: [FOCK_TILES_FP64] one molecule,
+    // FP64 kernels; [FOCK_TILES_MIXED] one molecule, mixed form (FP32 bulk kernels);
+    // [FOCK_TILES_BATCH] batched builds
+    int tiles[3][NUM_CLASSES];
+    int batch_schedule = FOCK_SCHEDULE_COST;  // block order of batched builds
     bool timing = false;
     hipEvent_t tev[NUM_CLASSES + 1] = {};
     fock_status fail(fock_status st, const std::string &msg) {
@@ -314,6 +322,8 @@ extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return FOCK_INVALID_ARG;
     ctx->device = device;
+    for (auto &form : ctx->tiles)
+        for (int &t : form) t = DEFAULT_TILE;
     HIPCHK(hipSetDevice(device));
     {
         int lds = 0;
@@ -387,6 +397,7 @@ extern "C" void fock_scf_opts_default(fock_scf_opts *o) {
     o->diis_window = 16;
     o->level_shift = 0.0;
     o->level_shift_until = 0.0;
+    o->autotune = 0;
 }
 
 // ====================================================================== Boys
@@ -437,6 +448,39 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
 }
 
 namespace {
+// Blocks of class cls at a tile: runs of up to `tile` consecutive tasks of one bra pair (task
+// indices class-local), most expensive first (load balance; stable for determinism).
+void make_class_blocks(const Prepared &pr, int cls, int tile, std::vector<Block> &out) {
+    const int t0 = pr.task_off[cls], t1 = pr.task_off[cls + 1];
+    const size_t first = out.size();
+    for (int q = t0; q < t1;) {
+        int r = q, cost = 0;
+        while (r < t1 && pr.h_tasks[r].bra == pr.h_tasks[q].bra && r - q < tile) {
+            cost += pr.h_tasks[r].flags * pr.h_tasks[r].ket_count;
+            ++r;
+        }
+        out.push_back(Block{q - t0, r - q, pr.h_tasks[q].bra, cost});
+        q = r;
+    }
+    std::stable_sort(out.begin() + first, out.end(), [](const Block &a, const Block &b) { return a.cost > b.cost; });
+}
+
+// Block lists of every class at the prepared tiles into d_blocks (room for tile 1 is kept, so
+// re-tiling never reallocates) and h_blocks.
+fock_status upload_blocks(fock_ctx *ctx, Prepared &pr) {
+    pr.h_blocks.clear();
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        pr.block_off[cls] = (int)pr.h_blocks.size();
+        make_class_blocks(pr, cls, pr.tile[cls], pr.h_blocks);
+    }
+    pr.block_off[NUM_CLASSES] = (int)pr.h_blocks.size();
+    HIPCHK(pr.d_blocks.ensure(std::max<size_t>(pr.h_tasks.size(), 1) * sizeof(Block)));
+    HIPCHK(hipMemcpyAsync(pr.d_blocks.p, pr.h_blocks.data(), pr.h_blocks.size() * sizeof(Block),
+                          hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));  // h_blocks may be rebuilt right after
+    return FOCK_OK;
+}
+
 // Shell pairs, primitive-pair data, screen lists, tasks and blocks of one basis into pr.
 fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *basis, int32_t policy, int32_t screen,
                          double tau, int32_t precision) {
@@ -689,7 +733,6 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     };
     mark("soa");
     std::vector<Task> tasks;
-    std::vector<Block> blocks_all;
     std::vector<int> gfirst_all, glast_all;
     fock_jk_stats st{};
     st.n_basis = n;
@@ -741,25 +784,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
             }
         }
         // tasks stay bra-major (the tasks of one bra pair, and of one (bra, bucket) group, are
-        // adjacent); blocks of up to BLOCK_TASKS tasks of one bra pair feed the blocked digestion,
-        // most expensive blocks first (load balance; stable for determinism)
+        // adjacent); blocks of up to a tile of tasks of one bra pair feed the blocked digestion
         const std::vector<int> &sg = cgid;
-        {
-            std::vector<Block> cb;
-            for (size_t q = 0; q < ct.size();) {
-                size_t r = q;
-                int cost = 0;
-                while (r < ct.size() && ct[r].bra == ct[q].bra && r - q < (size_t)BLOCK_TASKS) {
-                    cost += ct[r].flags * ct[r].ket_count;
-                    ++r;
-                }
-                cb.push_back(Block{(int)q, (int)(r - q), ct[q].bra, cost});
-                q = r;
-            }
-            std::stable_sort(cb.begin(), cb.end(), [](const Block &a, const Block &b) { return a.cost > b.cost; });
-            pr.block_off[cls] = (int)blocks_all.size();
-            blocks_all.insert(blocks_all.end(), cb.begin(), cb.end());
-        }
         std::vector<int> gf(ct.size()), gl(ct.size());
         for (size_t q = 0; q < ct.size(); ++q) gf[q] = (q > 0 && sg[q] == sg[q - 1]) ? gf[q - 1] : (int)q;
         for (size_t q = ct.size(); q-- > 0;) gl[q] = (q + 1 < ct.size() && sg[q] == sg[q + 1]) ? gl[q + 1] : (int)q;
@@ -772,11 +798,10 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         st.algorithmic_flop[cls] = (double)st.primitive_quartets[cls] * F_PRIM[cls] + 12.0 * (double)st.integrals[cls];
     }
     pr.task_off[NUM_CLASSES] = (int)tasks.size();
-    pr.block_off[NUM_CLASSES] = (int)blocks_all.size();
+    pr.h_tasks = tasks;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) pr.tile[cls] = ctx->tiles[precision][cls];
+    STCHK(upload_blocks(ctx, pr));
     mark("tasks_blocks");
-    HIPCHK(pr.d_blocks.ensure(std::max<size_t>(blocks_all.size(), 1) * sizeof(Block)));
-    HIPCHK(hipMemcpyAsync(pr.d_blocks.p, blocks_all.data(), blocks_all.size() * sizeof(Block), hipMemcpyHostToDevice,
-                          S(ctx)));
     if (screen == FOCK_SCREEN_REF_BF_PAIR) {
         // basis-function pair products within 1e-12 of the reference's threshold (estimate)
         std::vector<double> dv;
@@ -907,13 +932,14 @@ hipError_t scatter_pair_j(fock_ctx *ctx, double *Jb) {
                       pr.d_shell_nf.as<int>(), pr.d_Jk.as<double>(), Jb, pr.n, S(ctx));
 }
 
-// The blocked digestion applies when every class's workgroup fits the device's LDS.
-bool blocked_fits(fock_ctx *ctx, int n, bool fp32) {
+// The blocked digestion applies when every class's workgroup, at its tile, fits the device's LDS.
+bool blocked_fits(fock_ctx *ctx, int n, bool fp32, int policy, const int *tile) {
     if (n > BLOCKED_MAX_NBF) return false;
     for (int c = 0; c < NUM_CLASSES; ++c)
-        if (block_lds_bytes(c, n, ctx->jk.policy, fp32) > ctx->max_lds) return false;
+        if (block_lds_bytes(c, n, policy, fp32, tile[c]) > ctx->max_lds) return false;
     return true;
 }
+bool blocked_fits(fock_ctx *ctx, int n, bool fp32) { return blocked_fits(ctx, n, fp32, ctx->jk.policy, ctx->jk.tile); }
 
 // Kernel arguments shared by every build of class cls over the prepared task lists.
 JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *Jb, double *Kb) {
@@ -932,6 +958,7 @@ JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *J
     a.bt = ctx->bt;
     a.blocks = pr.d_blocks.as<Block>() + pr.block_off[cls];
     a.nblocks = pr.block_off[cls + 1] - pr.block_off[cls];
+    a.tile = pr.tile[cls];
     if (pr.prim_screen) {
         a.prim_q = pr.d_prim_q.as<float>();
         a.prim_thr = PRIM_THRESHOLD;
@@ -1075,6 +1102,137 @@ extern "C" fock_status fock_jk_build_device(fock_ctx *ctx, const double *dP, dou
     STCHK(enqueue_jk(ctx, dP, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), MODE_JK, nullptr));
     if (dJ) HIPCHK(symmetrize_add(pr.n, pr.d_Jb.as<double>(), dJ, S(ctx)));
     if (dK) HIPCHK(symmetrize_add(pr.n, pr.d_Kb.as<double>(), dK, S(ctx)));
+    return FOCK_OK;
+}
+
+namespace {
+// HIP events that are destroyed with the scope.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Best of three timed launches (after one untimed) of `launch` on the context's stream, in ms.
+template <typename F>
+fock_status time_launch(fock_ctx *ctx, EventPair &ev, F launch, double &best) {
+    if (!ev.a) HIPCHK(hipEventCreate(&ev.a));
+    if (!ev.b) HIPCHK(hipEventCreate(&ev.b));
+    STCHK(launch());
+    best = HUGE_VAL;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPCHK(hipEventRecord(ev.a, S(ctx)));
+        STCHK(launch());
+        HIPCHK(hipEventRecord(ev.b, S(ctx)));
+        HIPCHK(hipEventSynchronize(ev.b));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
+        best = std::min(best, (double)ms);
+    }
+    return FOCK_OK;
+}
+
+// Autotuner of the prepared J/K build: for each quartet class, the class kernel runs at every tile
+// of TILE_CHOICES whose workgroup fits the LDS, over the prepared task list at density dP; the
+// fastest tile is kept in the prepared block lists and in the context's table of the prepared
+// form (FP64 or mixed: the mixed form times its FP32 bulk kernels). The timing launches write
+// into the build's scratch J/K; the pair-indexed J accumulator is cleared afterwards.
+fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep) {
+    Prepared &pr = ctx->jk;
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool fp32 = pr.precision == 1;
+    const int n = pr.n;
+    for (int c = 0; c < NUM_CLASSES; ++c)
+        for (int k = 0; k < NUM_TILE_CHOICES; ++k) rep->ms[c][k] = -1.0;
+    rep->schedule = FOCK_SCHEDULE_COST;
+    rep->schedule_ms[0] = rep->schedule_ms[1] = -1.0;
+    if (!blocked_fits(ctx, n, fp32)) {  // the per-task kernels have no tile
+        for (int c = 0; c < NUM_CLASSES; ++c) rep->tile[c] = pr.tile[c];
+        rep->tune_ms = 0.0;
+        return FOCK_OK;
+    }
+    EventPair ev;
+    double *Jb = pr.d_Jb.as<double>(), *Kb = pr.d_Kb.as<double>();
+    const bool refscreen = pr.screen == FOCK_SCREEN_REF_BF_PAIR;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        if (pr.task_off[cls + 1] == pr.task_off[cls]) continue;
+        int best_tile = pr.tile[cls];
+        double best_ms = HUGE_VAL;
+        for (int k = 0; k < NUM_TILE_CHOICES; ++k) {
+            const int tile = TILE_CHOICES[k];
+            if (block_lds_bytes(cls, n, pr.policy, fp32, tile) > ctx->max_lds) continue;
+            pr.tile[cls] = tile;
+            STCHK(upload_blocks(ctx, pr));
+            const JKArgs a = class_args(ctx, cls, fp32, dP, Jb, Kb);
+            double ms = 0.0;
+            STCHK(time_launch(ctx, ev, [&]() -> fock_status {
+                HIPCHK(launch_eri_blocks(cls, a, pr.policy, refscreen, fp32, S(ctx)));
+                return FOCK_OK;
+            }, ms));
+            rep->ms[cls][k] = ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_tile = tile;
+            }
+        }
+        pr.tile[cls] = best_tile;
+    }
+    STCHK(upload_blocks(ctx, pr));
+    HIPCHK(fill_f64((int64_t)9 * (int64_t)pr.pairs.size(), 0.0, pr.d_Jk.as<double>(), S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    for (int c = 0; c < NUM_CLASSES; ++c) ctx->tiles[pr.precision][c] = rep->tile[c] = pr.tile[c];
+    rep->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return FOCK_OK;
+}
+
+bool valid_tile(int t) {
+    for (int k = 0; k < NUM_TILE_CHOICES; ++k)
+        if (TILE_CHOICES[k] == t) return true;
+    return false;
+}
+}  // namespace
+
+extern "C" fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
+    if (!P) return ctx->fail(FOCK_INVALID_ARG, "null density");
+    HIPCHK(hipSetDevice(ctx->device));
+    Prepared &pr = ctx->jk;
+    const size_t bytes = (size_t)pr.n * pr.n * sizeof(double);
+    HIPCHK(pr.d_P.ensure(bytes));
+    HIPCHK(hipMemcpyAsync(pr.d_P.p, P, bytes, hipMemcpyHostToDevice, S(ctx)));
+    fock_tune_report local{};
+    return tune_prepared(ctx, pr.d_P.as<double>(), report ? report : &local);
+}
+
+extern "C" fock_status fock_tiles_get(fock_ctx *ctx, int32_t form, int32_t tile[6], int32_t *schedule) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (form < 0 || form > 2) return ctx->fail(FOCK_INVALID_ARG, "unknown tile form");
+    if (tile)
+        for (int c = 0; c < NUM_CLASSES; ++c) tile[c] = ctx->tiles[form][c];
+    if (schedule) *schedule = form == FOCK_TILES_BATCH ? ctx->batch_schedule : FOCK_SCHEDULE_COST;
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_tiles_set(fock_ctx *ctx, int32_t form, const int32_t tile[6], int32_t schedule) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (form < 0 || form > 2) return ctx->fail(FOCK_INVALID_ARG, "unknown tile form");
+    if (schedule != FOCK_SCHEDULE_COST && schedule != FOCK_SCHEDULE_XCD)
+        return ctx->fail(FOCK_INVALID_ARG, "unknown batch schedule");
+    if (tile)
+        for (int c = 0; c < NUM_CLASSES; ++c)
+            if (!valid_tile(tile[c])) return ctx->fail(FOCK_INVALID_ARG, "tile must be 1, 2, 4, 8 or 16");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (tile)
+        for (int c = 0; c < NUM_CLASSES; ++c) ctx->tiles[form][c] = tile[c];
+    if (form == FOCK_TILES_BATCH) ctx->batch_schedule = schedule;
+    Prepared &pr = ctx->jk;
+    if (tile && pr.valid && form == pr.precision) {  // the prepared build follows at once
+        for (int c = 0; c < NUM_CLASSES; ++c) pr.tile[c] = tile[c];
+        STCHK(upload_blocks(ctx, pr));
+    }
     return FOCK_OK;
 }
 
@@ -1431,7 +1589,8 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         mols.push_back(std::move(bm));
     }
     phase_mark("prepare");
-    if (!blocked_fits(ctx, nmax, false)) return ctx->fail(FOCK_UNSUPPORTED, "largest basis too large for a batch");
+    if (!blocked_fits(ctx, nmax, false, opts.integral_policy, ctx->tiles[FOCK_TILES_BATCH]))
+        return ctx->fail(FOCK_UNSUPPORTED, "largest basis too large for a batch");
     // size groups: molecules of equal n share one batched eigensolver call
     std::map<int, std::vector<int>> groups;
     for (int m = 0; m < count; ++m) groups[mols[m]->n].push_back(m);
@@ -1522,29 +1681,77 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     for (auto &bp : mols) HIPCHK(gwh_guess(bp->n, bp->S.as<double>(), bp->H.as<double>(), bp->F.as<double>(), st));
     STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.F.as<double>(); }));
 
-    // batched Fock-build tables: one record per (molecule, class), blocks of every molecule
-    // sorted by cost (descending) per class
+    // batched Fock-build tables: one record per (molecule, class); per class, the blocks of every
+    // molecule (at the batch tiles) in the batch schedule
     std::vector<BatchRec> recs((size_t)count * NUM_CLASSES);
     std::vector<std::vector<int2>> bbl(NUM_CLASSES);
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
-        std::vector<std::pair<int, int2>> cb;  // (cost, (record, block))
-        for (int m = 0; m < count; ++m) {
-            Prepared &pr = mols[m]->pr;
-            const int nb = pr.block_off[cls + 1] - pr.block_off[cls];
-            std::vector<Block> hb(nb);
-            if (nb) HIPCHK(hipMemcpy(hb.data(), pr.d_blocks.as<Block>() + pr.block_off[cls], nb * sizeof(Block),
-                                     hipMemcpyDeviceToHost));
-            for (int i = 0; i < nb; ++i) cb.push_back({hb[i].cost, make_int2(m * NUM_CLASSES + cls, i)});
-        }
-        std::stable_sort(cb.begin(), cb.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
-        for (auto &e : cb) bbl[cls].push_back(e.second);
-    }
     DevBuf d_recs, d_bbl[NUM_CLASSES], d_scat;
     HIPCHK(d_recs.ensure(recs.size() * sizeof(BatchRec)));
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
-        HIPCHK(d_bbl[cls].ensure(std::max<size_t>(bbl[cls].size(), 1) * sizeof(int2)));
-        HIPCHK(hipMemcpy(d_bbl[cls].p, bbl[cls].data(), bbl[cls].size() * sizeof(int2), hipMemcpyHostToDevice));
+    int tiles[NUM_CLASSES];
+    for (int c = 0; c < NUM_CLASSES; ++c) tiles[c] = ctx->tiles[FOCK_TILES_BATCH][c];
+    int schedule = ctx->batch_schedule;
+    // XCD-affine schedule: molecules over the 8 XCDs by total block cost (largest first onto the
+    // least loaded XCD), so each XCD's L2 holds the data of its own molecules only
+    constexpr int NXCD = 8;
+    std::vector<int> xcd_of(count, 0);
+    {
+        std::vector<std::pair<int64_t, int>> tot(count);
+        for (int m = 0; m < count; ++m) {
+            int64_t c = 0;
+            for (const Task &t : mols[m]->pr.h_tasks) c += (int64_t)t.flags * t.ket_count;
+            tot[m] = {c, m};
+        }
+        std::stable_sort(tot.begin(), tot.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        int64_t load[NXCD] = {0};
+        for (auto &e : tot) {
+            const int x = (int)(std::min_element(load, load + NXCD) - load);
+            xcd_of[e.second] = x;
+            load[x] += e.first;
+        }
     }
+    // per-molecule block lists at `tiles` (the records point into them)
+    auto set_tiles = [&]() -> fock_status {
+        for (auto &bp : mols) {
+            for (int c = 0; c < NUM_CLASSES; ++c) bp->pr.tile[c] = tiles[c];
+            STCHK(upload_blocks(ctx, bp->pr));
+        }
+        return FOCK_OK;
+    };
+    // the launch order of class cls's blocks: by cost over all molecules, or XCD-affine (workgroup
+    // i runs on XCD i mod 8: slot 8 j + x holds the j-th costliest block of XCD x's molecules, a
+    // record of -1 pads the shorter lists)
+    auto make_schedule = [&](int cls) -> fock_status {
+        bbl[cls].clear();
+        if (schedule == FOCK_SCHEDULE_XCD) {
+            std::vector<std::pair<int, int2>> lists[NXCD];
+            for (int m = 0; m < count; ++m) {
+                const Prepared &pr = mols[m]->pr;
+                for (int i = pr.block_off[cls]; i < pr.block_off[cls + 1]; ++i)
+                    lists[xcd_of[m]].push_back({pr.h_blocks[i].cost, make_int2(m * NUM_CLASSES + cls, i - pr.block_off[cls])});
+            }
+            size_t len = 0;
+            for (auto &l : lists) {
+                std::stable_sort(l.begin(), l.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+                len = std::max(len, l.size());
+            }
+            for (size_t j = 0; j < len; ++j)
+                for (auto &l : lists) bbl[cls].push_back(j < l.size() ? l[j].second : make_int2(-1, 0));
+        } else {
+            std::vector<std::pair<int, int2>> cb;  // (cost, (record, block))
+            for (int m = 0; m < count; ++m) {
+                const Prepared &pr = mols[m]->pr;
+                for (int i = pr.block_off[cls]; i < pr.block_off[cls + 1]; ++i)
+                    cb.push_back({pr.h_blocks[i].cost, make_int2(m * NUM_CLASSES + cls, i - pr.block_off[cls])});
+            }
+            std::stable_sort(cb.begin(), cb.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+            for (auto &e : cb) bbl[cls].push_back(e.second);
+        }
+        HIPCHK(hipStreamSynchronize(st));  // no launch still reads the old list
+        HIPCHK(d_bbl[cls].ensure(std::max<size_t>(bbl[cls].size(), 1) * sizeof(int2)));
+        HIPCHK(hipMemcpyAsync(d_bbl[cls].p, bbl[cls].data(), bbl[cls].size() * sizeof(int2), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return FOCK_OK;
+    };
     std::vector<ScatterRec> scat(count);
     int max_npair = 0;
     for (int m = 0; m < count; ++m) {
@@ -1583,9 +1790,73 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         HIPCHK(hipStreamSynchronize(st));  // recs is rewritten on the host next time
         return FOCK_OK;
     };
-    STCHK(upload_records());
-    phase_mark("guess_tables");
     const bool refscreen = opts.screen == FOCK_SCREEN_REF_BF_PAIR;
+    // one class launch over the whole batch
+    auto launch_class = [&](int cls) -> fock_status {
+        if (bbl[cls].empty()) return FOCK_OK;
+        JKArgs a{};
+        a.nbf = nmax;
+        a.nblocks = (int)bbl[cls].size();
+        a.tile = tiles[cls];
+        a.bt = ctx->bt;
+        a.dthr = mols[0]->pr.dthr;
+        a.prim_thr = PRIM_THRESHOLD;
+        a.batch = d_recs.as<BatchRec>() + 0;
+        a.bblocks = d_bbl[cls].as<int2>();
+        HIPCHK(launch_eri_blocks(cls, a, opts.integral_policy, refscreen, false, st));
+        return FOCK_OK;
+    };
+    auto fits = [&](int cls, int tile) {
+        return block_lds_bytes(cls, nmax, opts.integral_policy, false, tile) <= ctx->max_lds;
+    };
+    if (opts.autotune) {
+        // every candidate tile of every class under both schedules at the guess densities; the
+        // schedule with the smaller sum of per-class best times wins, then each class's best tile
+        // under it. The timing launches write into the build's scratch (J, K cleared by every
+        // build; the pair-indexed J accumulators are cleared here).
+        EventPair ev;
+        double best[2][NUM_CLASSES];
+        int best_tile[2][NUM_CLASSES];
+        for (int sc = 0; sc < 2; ++sc)
+            for (int c = 0; c < NUM_CLASSES; ++c) {
+                best[sc][c] = HUGE_VAL;
+                best_tile[sc][c] = tiles[c];
+            }
+        for (int k = 0; k < NUM_TILE_CHOICES; ++k) {
+            const int tile = TILE_CHOICES[k];
+            for (int c = 0; c < NUM_CLASSES; ++c) tiles[c] = fits(c, tile) ? tile : ctx->tiles[FOCK_TILES_BATCH][c];
+            STCHK(set_tiles());
+            STCHK(upload_records());
+            for (int sc = 0; sc < 2; ++sc) {
+                schedule = sc;
+                for (int c = 0; c < NUM_CLASSES; ++c) {
+                    if (!fits(c, tile)) continue;
+                    STCHK(make_schedule(c));
+                    if (bbl[c].empty()) continue;
+                    double ms = 0.0;
+                    STCHK(time_launch(ctx, ev, [&]() { return launch_class(c); }, ms));
+                    if (ms < best[sc][c]) {
+                        best[sc][c] = ms;
+                        best_tile[sc][c] = tile;
+                    }
+                }
+            }
+        }
+        double tot[2] = {0.0, 0.0};
+        for (int sc = 0; sc < 2; ++sc)
+            for (int c = 0; c < NUM_CLASSES; ++c)
+                if (best[sc][c] < HUGE_VAL) tot[sc] += best[sc][c];
+        schedule = tot[1] < tot[0] ? FOCK_SCHEDULE_XCD : FOCK_SCHEDULE_COST;
+        for (int c = 0; c < NUM_CLASSES; ++c) ctx->tiles[FOCK_TILES_BATCH][c] = tiles[c] = best_tile[schedule][c];
+        ctx->batch_schedule = schedule;
+        for (auto &bp : mols)
+            HIPCHK(fill_f64((int64_t)9 * (int64_t)bp->pr.pairs.size(), 0.0, bp->pr.d_Jk.as<double>(), st));
+        phase_mark("autotune");
+    }
+    STCHK(set_tiles());
+    STCHK(upload_records());
+    for (int c = 0; c < NUM_CLASSES; ++c) STCHK(make_schedule(c));
+    phase_mark("guess_tables");
     std::vector<double> hostd((size_t)count * 18);
     int active = count;
     // G(P), F = H + G of the whole batch: one launch per quartet class; converged molecules'
@@ -1599,18 +1870,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Jb.as<double>(), st));
             HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Kb.as<double>(), st));
         }
-        for (int cls = 0; cls < NUM_CLASSES; ++cls) {
-            if (bbl[cls].empty()) continue;
-            JKArgs a{};
-            a.nbf = nmax;
-            a.nblocks = (int)bbl[cls].size();
-            a.bt = ctx->bt;
-            a.dthr = mols[0]->pr.dthr;
-            a.prim_thr = PRIM_THRESHOLD;
-            a.batch = d_recs.as<BatchRec>() + 0;
-            a.bblocks = d_bbl[cls].as<int2>();
-            HIPCHK(launch_eri_blocks(cls, a, opts.integral_policy, refscreen, false, st));
-        }
+        for (int cls = 0; cls < NUM_CLASSES; ++cls) STCHK(launch_class(cls));
         HIPCHK(scatter_jk_batch(count, d_scat.as<ScatterRec>(), max_npair, st));
         for (auto &bp : mols)
             HIPCHK(fock_from_jk(bp->n, bp->H.as<double>(), bp->pr.d_Jb.as<double>(), bp->pr.d_Kb.as<double>(),
@@ -1963,32 +2223,37 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     // trace-correcting purification (Niklasson, Phys. Rev. B 66, 155115 (2002)). From
     // X0 = (hi - F')/(hi - lo) (Gershgorin bounds) each step takes X^2 or 2X - X^2, whichever
     // trace is closer to nocc, until tr X - tr X^2 = sum lambda (1 - lambda) vanishes; then
-    // P = 2 X Xp X^T. False (and no P) when it does not converge, e.g. without a gap at the
-    // Fermi level.
+    // P = 2 X Xp X^T. False (and no P) when it does not converge within MAX_STEPS (about twice
+    // log2(width / gap) steps are needed: a gap of 1e-4 of the width takes ~50, a gap at rounding
+    // level over 100), and when the whole
+    // spectrum is degenerate to working precision (width below 1e-8 of max|lambda|: the bounds
+    // would only measure rounding noise, e.g. the GWH guess of H2, a multiple of the identity).
+    // Those go to the eigensolver, whose choice within a degenerate space is the reference's.
     DevBuf dXp, dX2, dPur;
     const bool purify_ok = opts.integral_policy == FOCK_POLICY_ACCURATE;
     if (purify_ok) {
         HIPCHK(dXp.ensure(bytes));
         HIPCHK(dX2.ensure(bytes));
-        HIPCHK(dPur.ensure(4 * sizeof(double)));
+        HIPCHK(dPur.ensure(8 * sizeof(double)));
     }
     auto purify = [&](bool &ok) -> fock_status {
         ok = false;
-        double *Xp = dXp.as<double>(), *X2 = dX2.as<double>(), *pur = dPur.as<double>();  // pur: lo, hi | trX, trX2, grow
-        HIPCHK(gershgorin_bounds(n, T2, pur, S(ctx)));
-        HIPCHK(tc2_init(n, T2, pur, Xp, S(ctx)));
+        double *Xp = dXp.as<double>(), *X2 = dX2.as<double>(), *pur = dPur.as<double>();  // pur: trX, trX2, grow, -, lo, hi
+        HIPCHK(gershgorin_bounds(n, T2, pur + 4, S(ctx)));
+        HIPCHK(tc2_init(n, T2, pur + 4, Xp, S(ctx)));
         // steps run in chunks without host round trips (the grow/shrink choice is made on the
         // device); the host looks at tr X - tr X^2 after each chunk
-        constexpr int CHUNK = 8;
-        for (int it = 0; it < 96 && !ok; it += CHUNK) {
+        constexpr int CHUNK = 8, MAX_STEPS = 64;
+        for (int it = 0; it < MAX_STEPS && !ok; it += CHUNK) {
             for (int k = 0; k < CHUNK; ++k) {
                 HIPCHK(gemm_f64(false, false, n, n, n, 1.0, Xp, n, Xp, n, 0.0, X2, n, S(ctx)));
                 HIPCHK(trace2(n, Xp, X2, (double)nocc, pur, S(ctx)));
                 HIPCHK(tc2_update((int64_t)n * n, pur + 2, Xp, X2, S(ctx)));
             }
-            double h[3];
-            HIPCHK(hipMemcpyAsync(h, pur, 3 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            double h[6];
+            HIPCHK(hipMemcpyAsync(h, pur, 6 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
+            if (!(h[5] - h[4] > 1e-8 * std::max(std::fabs(h[4]), std::fabs(h[5])))) return FOCK_OK;
             const double idem = h[0] - h[1];  // of the step before the last update
             if (!std::isfinite(idem)) return FOCK_OK;
             // quadratic convergence: below 1e-9 before the last step means below 1e-18 after it
@@ -2031,6 +2296,11 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     }
 
     phase_mark("guess");
+    if (opts.autotune) {  // tiles for this molecule at the guess density, kept in the context
+        fock_tune_report rep{};
+        STCHK(tune_prepared(ctx, P_, &rep));
+        phase_mark("autotune");
+    }
     // mixed form. Phase 0 (bulk): FP32 quartets; with incremental builds G(P_k) = G(P_{k-1}) +
     // G(P_k - P_{k-1}) over the density-weighted screen, all in FP32. Phase 1: without incremental
     // builds, FP64 full builds; with them, an FP64 full build that resets the accumulated G, then

@@ -33,6 +33,7 @@ struct JKArgs {
     const int *d_ntasks;    // compacted tasks: number of valid tasks on the device (null: ntasks)
     const Block *blocks;    // blocked digestion: groups of tasks sharing one bra pair
     int nblocks;
+    int tile;               // blocked digestion: most tasks per block (LDS is sized for it)
     // density-weighted screening (incremental builds): a lane is evaluated only when
     // Q_ab Q_cd max(2|P_ab|, 2|P_cd|, |P_ac|, |P_ad|, |P_bc|, |P_bd|) lies in [bound_lo, bound_hi)
     const int2 *pair_shell;  // null: no density screening
@@ -82,10 +83,14 @@ struct BatchRec {
 constexpr int PRIM_FIELDS = 13;
 // largest primitive-pair count of a shell pair (contractions of up to 6 primitives)
 constexpr int MAX_PRIM_PAIRS = 36;
-// tasks of one bra pair per workgroup of the blocked digestion (one wave per task at a time)
-constexpr int BLOCK_TASKS = 8;
+// tasks of one bra pair per workgroup of the blocked digestion (one wave per task at a time):
+// the tile, chosen per quartet class from TILE_CHOICES (default DEFAULT_TILE) by the autotuner
+constexpr int DEFAULT_TILE = 8;
+constexpr int MAX_TILE = 16;
+constexpr int NUM_TILE_CHOICES = 5;
+constexpr int TILE_CHOICES[NUM_TILE_CHOICES] = {1, 2, 4, 8, 16};
 // largest basis for the blocked digestion: (3 + 3) K rows of n doubles in LDS next to the staged
-// Boys slab (at most 33 KB) and the block's ket records (13 KB)
+// Boys slab (at most 33 KB) and the block's ket records (1.6 KB per task of the tile)
 constexpr int BLOCKED_MAX_NBF = 600;
 
 // density-weighted screening of the static task list (compact.hip)
@@ -195,7 +200,7 @@ hipError_t launch_prim_count(const PrimCountArgs &a, hipStream_t s);
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);
 hipError_t launch_eri_blocks(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s);
 // dynamic LDS of one workgroup of the blocked kernel of class cls
-size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32);
+size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile);
 
 // one_electron.hip
 struct OneElArgs {

@@ -32,6 +32,12 @@ SCREEN_SCHWARZ = 1
 SCREEN_NONE = 2
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+TILES_FP64 = 0
+TILES_MIXED = 1
+TILES_BATCH = 2
+SCHEDULE_COST = 0
+SCHEDULE_XCD = 1
+TILE_CHOICES = (1, 2, 4, 8, 16)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -41,6 +47,7 @@ EXPORTED = [
     "fock_scf_opts_default", "fock_scf_run", "fock_rhf_gradient", "fock_scf_batch_run", "fock_boys", "fock_eri_dense", "fock_one_electron",
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
     "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
+    "fock_jk_autotune", "fock_tiles_get", "fock_tiles_set",
 ]
 
 
@@ -59,7 +66,8 @@ class ScfOpts(ctypes.Structure):
                 ("schwarz_threshold", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("incremental", ctypes.c_int32), ("initial_density", _dp), ("diis_start", ctypes.c_int32),
                 ("diis_window", ctypes.c_int32), ("level_shift", ctypes.c_double),
-                ("level_shift_until", ctypes.c_double), ("reserved", ctypes.c_int32 * 4)]
+                ("level_shift_until", ctypes.c_double), ("autotune", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class ScfIter(ctypes.Structure):
@@ -81,6 +89,11 @@ class JKStats(ctypes.Structure):
                 ("shell_quartets", ctypes.c_int64 * 6), ("primitive_quartets", ctypes.c_int64 * 6),
                 ("integrals", ctypes.c_int64 * 6), ("tasks", ctypes.c_int64 * 6),
                 ("algorithmic_flop", ctypes.c_double * 6), ("near_threshold", ctypes.c_int64)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int32 * 6), ("ms", (ctypes.c_double * 5) * 6), ("schedule", ctypes.c_int32),
+                ("schedule_ms", ctypes.c_double * 2), ("tune_ms", ctypes.c_double)]
 
 
 class FockError(RuntimeError):
@@ -127,6 +140,9 @@ def load(path: Optional[str] = None):
         "fock_diis": (ctypes.c_int, [vp, i32, i64, _dp, _dp, _dp]),
         "fock_jk_set_timing": (ctypes.c_int, [vp, i32]),
         "fock_jk_get_timings": (ctypes.c_int, [vp, _dp]),
+        "fock_jk_autotune": (ctypes.c_int, [vp, _dp, ctypes.POINTER(TuneReport)]),
+        "fock_tiles_get": (ctypes.c_int, [vp, i32, _ip, _ip]),
+        "fock_tiles_set": (ctypes.c_int, [vp, i32, _ip, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -229,6 +245,25 @@ class Context:
         ms = np.zeros(6)
         self.check(self.lib.fock_jk_get_timings(self.ptr, _d(ms)))
         return ms
+
+    def jk_autotune(self, P: np.ndarray) -> dict:
+        """Tile autotuner on the prepared build at density P: the chosen tile per quartet class
+        and the best time (ms) of every candidate tile (-1: not timed)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        rep = TuneReport()
+        self.check(self.lib.fock_jk_autotune(self.ptr, _d(P), ctypes.byref(rep)))
+        return {"tile": list(rep.tile), "ms": np.array([list(r) for r in rep.ms]), "tune_ms": rep.tune_ms}
+
+    def tiles(self, form: int):
+        """(tile per quartet class, batch schedule) of a form (TILES_*)."""
+        t = np.zeros(6, dtype=np.int32)
+        sc = ctypes.c_int32(0)
+        self.check(self.lib.fock_tiles_get(self.ptr, form, _i(t), ctypes.byref(sc)))
+        return [int(x) for x in t], sc.value
+
+    def set_tiles(self, form: int, tiles=None, schedule: int = SCHEDULE_COST):
+        t = None if tiles is None else np.ascontiguousarray(tiles, dtype=np.int32)
+        self.check(self.lib.fock_tiles_set(self.ptr, form, None if t is None else _i(t), schedule))
 
     def jk_stats(self) -> dict:
         st = JKStats()

@@ -114,7 +114,11 @@ typedef struct fock_scf_opts {
      * max|FDS - SDF| > level_shift_until (0: none, as the reference). */
     double level_shift;
     double level_shift_until;
-    int32_t reserved[4];
+    /* Nonzero: fock_scf_run / fock_scf_batch_run run the tile autotuner (fock_jk_autotune) at the
+     * first iteration's density before iterating; the chosen tiles stay in the context for later
+     * calls. 0 (default): the context's current tiles. */
+    int32_t autotune;
+    int32_t reserved[3];
 } fock_scf_opts;
 
 /* Per-iteration record (optional output). */
@@ -263,6 +267,41 @@ typedef struct fock_jk_stats {
     int64_t near_threshold;      /* REF_BF_PAIR: |estimate - 1e-6| < 1e-12 pairs met */
 } fock_jk_stats;
 fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats);
+
+/* ------------------------------------------------------------------ autotuner */
+
+/* The blocked J/K digestion runs one workgroup per block of up to `tile` wavefront tasks (64 ket
+ * pairs each) of one bra pair; the tile sets the workgroup's LDS and so its occupancy, against
+ * the per-block cost of staging the bra pair, the Boys slab and the K rows. Tiles are chosen per
+ * quartet class and per form; batched builds also choose the block schedule. The reference has no
+ * counterpart (its builds are CPU loops); these entry points serve the batching form. */
+typedef enum fock_tile_form {
+    FOCK_TILES_FP64 = 0,   /* one molecule, FP64 kernels */
+    FOCK_TILES_MIXED = 1,  /* one molecule, mixed form (FP32 bulk kernels) */
+    FOCK_TILES_BATCH = 2   /* fock_scf_batch_run */
+} fock_tile_form;
+typedef enum fock_batch_schedule {
+    FOCK_SCHEDULE_COST = 0,  /* the blocks of all molecules by cost, descending */
+    FOCK_SCHEDULE_XCD = 1    /* each molecule's blocks on one XCD (L2) of the device: workgroup i
+                              * runs on XCD i mod 8, molecules are balanced over the XCDs by cost */
+} fock_batch_schedule;
+typedef struct fock_tune_report {
+    int32_t tile[6];         /* chosen tile per quartet class */
+    double ms[6][5];         /* per class and candidate tile 1, 2, 4, 8, 16: best of three
+                              * launches in ms (batched: under the chosen schedule); -1 not timed */
+    int32_t schedule;        /* batched builds: chosen fock_batch_schedule */
+    double schedule_ms[2];   /* batched builds: sum over classes of the best time per schedule */
+    double tune_ms;          /* wall time of the tuning */
+} fock_tune_report;
+
+/* Time every candidate tile of each class on the prepared J/K build (fock_jk_prepare) at density
+ * P (host, row-major n x n), keep the fastest for the prepared build and for this context's later
+ * builds of the prepared form (FP64 or mixed). report may be NULL. */
+fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report);
+/* Read or set the context's tiles of a form (fock_tile_form); tiles are 1, 2, 4, 8 or 16 (default
+ * 8). schedule applies to FOCK_TILES_BATCH (fock_batch_schedule; default COST). tile may be NULL. */
+fock_status fock_tiles_get(fock_ctx *ctx, int32_t form, int32_t tile[6], int32_t *schedule);
+fock_status fock_tiles_set(fock_ctx *ctx, int32_t form, const int32_t tile[6], int32_t schedule);
 
 #ifdef __cplusplus
 }

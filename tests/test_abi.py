@@ -36,6 +36,7 @@ def test_default_options_are_the_reference_constants():
     assert o.integral_policy == abi.POLICY_REF_COMPAT and o.screen == abi.SCREEN_REF_BF_PAIR
     assert o.precision == abi.PRECISION_FP64 and o.incremental == 0
     assert o.diis_start == 5 and o.diis_window == 16 and o.level_shift == 0.0  # scf/src/lib.rs:197-207
+    assert o.autotune == 0
 
 
 def test_struct_sizes_match_header_layout():
@@ -45,6 +46,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(abi.ScfOpts) == 96
     assert ctypes.sizeof(abi.ScfIter) == 32
     assert ctypes.sizeof(abi.JKStats) == 12 + 4 + 6 * 8 * 5 + 8
+    assert ctypes.sizeof(abi.TuneReport) == 6 * 4 + 6 * 5 * 8 + 4 + 4 + 2 * 8 + 8
 
 
 def test_component_order_is_the_references():
