@@ -141,7 +141,9 @@ def batch_form(ctx, abi, m, np):
     mols = [m.flatten(m.build(g, "6-31G")) for g in m.water_cluster_batch(64, seed=20261202)]
     o = abi.default_opts(max_iters=200, epsilon=SCF_EPS, integral_policy=abi.POLICY_ACCURATE,
                          screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12, **SCF_SCHEDULE)
-    ctx.scf_batch_run(mols, o)  # warm-up (code objects, allocations)
+    o.autotune = 1
+    ctx.scf_batch_run(mols, o)  # warm-up (code objects, allocations) and tile/schedule tuning
+    o.autotune = 0
     t0 = time.time()
     st, statuses, res = ctx.scf_batch_run(mols, o)
     t_batch = time.time() - t0
@@ -154,6 +156,8 @@ def batch_form(ctx, abi, m, np):
         "speedup": t_seq / t_batch, "scf_per_s": 64 / t_batch,
         "max_abs_dE_batch_vs_single": float(max(abs(a["total_energy"] - b["total_energy"]) for a, b in zip(res, single))),
         "n_basis_range": [min(fb.n for fb, _ in mols), max(fb.n for fb, _ in mols)],
+        "tiles": ctx.tiles(abi.TILES_BATCH)[0],
+        "schedule": ["cost", "xcd"][ctx.tiles(abi.TILES_BATCH)[1]],
     }
 
 
@@ -179,6 +183,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true", help="skip both SCF runs (builds use a synthetic density)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile (8 tasks per workgroup)")
+    ap.add_argument("--tiles", default="", metavar="JSON",
+                    help='set the tiles instead of tuning, e.g. the "tiles" of an earlier run\'s config: '
+                         '{"fp64": {"tile": [...]}, "mixed": {"tile": [...]}} (profiling runs)')
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -214,11 +222,30 @@ def main():
     ctx.set_stream(stream.cuda_stream)
     result = {}
 
+    # tile autotuning (setup, once per form, kept in the context for every later prepare): a full
+    # build's work does not depend on the density values, so a synthetic density serves
+    tiles = {}
+    if args.tiles:
+        tiles = json.loads(args.tiles)
+        for name, form in (("fp64", abi.TILES_FP64), ("mixed", abi.TILES_MIXED)):
+            ctx.set_tiles(form, tiles[name]["tile"])
+    elif not args.no_autotune:
+        for name, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
+            ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12, prec)
+            rep = ctx.jk_autotune(np.eye(n) * 0.1)
+            tiles[name] = {"tile": rep["tile"], "tune_ms": round(rep["tune_ms"], 1)}
+            log("tiles %s: %s (%.0f ms)" % (name, rep["tile"], rep["tune_ms"]))
+
     # reference density: the FP64 SCF (accurate, Schwarz 1e-12) -- also the FP64 SCF wall time
     gold_path = os.path.join(REPO, "tests", "golden", "e264.npz")
     e_oracle = float(np.load(gold_path)["e264_acc_energy"][0]) if os.path.exists(gold_path) else None
     scf = {}
     P_ref = None
+    if not args.no_scf:  # untimed warm-up of both forms (code objects, solver workspaces)
+        for prec, incr in ((abi.PRECISION_FP64, 0), (abi.PRECISION_MIXED, 1)):
+            ctx.scf_run(fb, fn, abi.default_opts(max_iters=2, epsilon=SCF_EPS, integral_policy=abi.POLICY_ACCURATE,
+                                                 screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12,
+                                                 precision=prec, incremental=incr, **SCF_SCHEDULE))
     for name, prec, incr in (("fp64", abi.PRECISION_FP64, 0), ("mixed", abi.PRECISION_MIXED, 1)):
         if args.no_scf:
             continue
@@ -282,7 +309,8 @@ def main():
         "config": {"workload": "C20H42/6-31G direct Fock build, n=264, accurate integrals, Schwarz 1e-12",
                    "n_basis": n, "shells": mixed["stats"]["n_shells"],
                    "shell_quartets": int(sum(mixed["stats"]["shell_quartets"])),
-                   "primitive_quartets": int(sum(mixed["stats"]["primitive_quartets"]))},
+                   "primitive_quartets": int(sum(mixed["stats"]["primitive_quartets"])),
+                   "tiles": tiles or "default 8"},
         "fp64": {"builds_per_s": 1e3 / fp64["ms"], "ms_per_step": fp64["ms"],
                  "dominant_kernel": CLASS_NAMES[fp64_dom], "dominant_achieved_tflops": fp64_ach,
                  "dominant_frac": fp64_ach / PEAK_FP64},
