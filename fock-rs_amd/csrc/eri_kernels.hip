@@ -130,16 +130,40 @@ __device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const Pai
     if (args.prim_q) {
         // primitive pairs are in descending q (bra and ket): the kept bra primitives of a ket
         // primitive form a prefix, and the wave stops at the first (k, i) no lane keeps
-        for (int k = 0; k < ket.K; ++k) {
-            const float qk = active ? (float)kbase[((size_t)k * PRIM_FIELDS + 12) * ket_stride] : 0.0f;
-            if (__ballot(qk * bra_q[0] > args.prim_thr) == 0) break;
-            const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
-            for (int i = 0; i < bra.K; ++i) {
-                const bool keep = bra_q[i] * qk > args.prim_thr;
-                if (__ballot(keep) == 0) break;
-                if (keep) {
-                    const PrimPair<Real> b = bra_pp[i];
-                    accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+        if constexpr (sizeof(Real) == 4) {
+            // FP32: ket primitive k + 1 is loaded while primitive k is evaluated (every pair has
+            // K >= 1); in FP64 the extra registers cost the larger classes more than it saves
+            float qn = active ? (float)kbase[(size_t)12 * ket_stride] : 0.0f;
+            PrimPair<Real> kn = soa_prim(kbase, 0, ket_stride);
+            for (int k = 0; k < ket.K; ++k) {
+                const float qk = qn;
+                const PrimPair<Real> kq = kn;
+                if (k + 1 < ket.K) {
+                    qn = active ? (float)kbase[((size_t)(k + 1) * PRIM_FIELDS + 12) * ket_stride] : 0.0f;
+                    kn = soa_prim(kbase, k + 1, ket_stride);
+                }
+                if (__ballot(qk * bra_q[0] > args.prim_thr) == 0) break;
+                for (int i = 0; i < bra.K; ++i) {
+                    const bool keep = bra_q[i] * qk > args.prim_thr;
+                    if (__ballot(keep) == 0) break;
+                    if (keep) {
+                        const PrimPair<Real> b = bra_pp[i];
+                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+                    }
+                }
+            }
+        } else {
+            for (int k = 0; k < ket.K; ++k) {
+                const float qk = active ? (float)kbase[((size_t)k * PRIM_FIELDS + 12) * ket_stride] : 0.0f;
+                if (__ballot(qk * bra_q[0] > args.prim_thr) == 0) break;
+                const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
+                for (int i = 0; i < bra.K; ++i) {
+                    const bool keep = bra_q[i] * qk > args.prim_thr;
+                    if (__ballot(keep) == 0) break;
+                    if (keep) {
+                        const PrimPair<Real> b = bra_pp[i];
+                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
+                    }
                 }
             }
         }
