@@ -35,13 +35,15 @@ const double REF_LUT_HOST[200] = {
 #include "ref_boys_lut.inc"
 };
 
-// tasks (64 kets each) per workgroup of the blocked digestion
-
 // f_prim of the canonical McMurchie-Davidson count per quartet class (see fock_jk_stats)
 constexpr double F_PRIM[NUM_CLASSES] = {38, 85, 266, 298, 977, 3833};
 // primitive screen of the accurate policy: a primitive quartet whose Cauchy-Schwarz bound
 // q_ij q_kl (contraction coefficients included) is at most this is not evaluated
 constexpr float PRIM_THRESHOLD = 1e-15f;
+// the same screen for the FP32 quartet kernels of the mixed form: their integrals carry a relative
+// rounding error of ~1e-7 of the quartet's bound, so primitive quartets bounded by 1e-11 are far
+// below it (the result of the mixed form always comes from an FP64 build)
+constexpr float PRIM_THRESHOLD32 = 1e-11f;
 // mixed-precision gradient: derivative integrals of tasks whose Schwarz bound Q_ab Q_cd (largest
 // ket of the task) is below this run in FP32, the others in FP64
 constexpr float GRAD_TAU64 = 1e-4f;
@@ -864,7 +866,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
 }
 
 // Primitive quartets the primitive screen keeps, per class (statistics and the algorithmic flop
-// count of the accurate policy); one pass over the task lists.
+// count of the accurate policy), at the threshold of the prepared precision's full builds (FP32
+// kernels for the mixed form); one pass over the task lists.
 fock_status count_kept_primitives(fock_ctx *ctx) {
     Prepared &pr = ctx->jk;
     if (pr.prim_counted) return FOCK_OK;
@@ -873,7 +876,8 @@ fock_status count_kept_primitives(fock_ctx *ctx) {
         HIPCHK(fill_f64(NUM_CLASSES, 0.0, pr.d_count.as<double>(), S(ctx)));  // all-zero bits
         for (int cls = 0; cls < NUM_CLASSES; ++cls) {
             PrimCountArgs ca{pr.d_tasks.as<Task>() + pr.task_off[cls], pr.task_off[cls + 1] - pr.task_off[cls],
-                             pr.d_pairs.as<PairInfo>(), pr.d_prim_q.as<float>(), PRIM_THRESHOLD,
+                             pr.d_pairs.as<PairInfo>(), pr.d_prim_q.as<float>(),
+                             pr.precision == 1 ? PRIM_THRESHOLD32 : PRIM_THRESHOLD,
                              pr.d_count.as<unsigned long long>() + cls};
             HIPCHK(launch_prim_count(ca, S(ctx)));
         }
@@ -961,7 +965,7 @@ JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *J
     a.tile = pr.tile[cls];
     if (pr.prim_screen) {
         a.prim_q = pr.d_prim_q.as<float>();
-        a.prim_thr = PRIM_THRESHOLD;
+        a.prim_thr = fp32 ? PRIM_THRESHOLD32 : PRIM_THRESHOLD;
     }
     a.soa = fp32 ? pr.d_soa32.p : pr.d_soa64.p;
     a.soa_pairs = pr.d_soa_pairs.as<PairInfo>();

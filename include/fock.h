@@ -255,7 +255,9 @@ fock_status fock_jk_set_timing(fock_ctx *ctx, int32_t enable);
 fock_status fock_jk_get_timings(fock_ctx *ctx, double ms[6]);
 
 /* Statistics of the prepared J/K build, per quartet class (ss|ss), (ps|ss), (ps|ps), (pp|ss),
- * (pp|ps), (pp|pp). Counts are of work that survives the screen. */
+ * (pp|ps), (pp|pp). Counts are of work that survives the screen (primitive quartets: the
+ * accurate policy's primitive screen at the threshold of the prepared precision's full builds,
+ * 1e-15 FP64, 1e-11 for the FP32 kernels of the mixed form). */
 typedef struct fock_jk_stats {
     int32_t n_basis, n_shells, n_shell_pairs;
     int64_t shell_quartets[6];
