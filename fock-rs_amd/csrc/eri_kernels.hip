@@ -490,6 +490,8 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
             pm = fmaxf(pm, fmaxf(Pm[ab.y * ns + cd.x], Pm[ab.y * ns + cd.y]));
             const float bound = args.pairQ[blk.bra] * args.pairQ[ket_idx] * pm;
             keep = bound >= args.bound_lo && bound < args.bound_hi;
+        } else if (args.ket_tau > 0.0f && keep) {
+            keep = args.pairQ[blk.bra] * args.pairQ[tk.ket_begin + lane] >= args.ket_tau;
         }
         const unsigned long long m = __ballot(keep);
         if (lane == 0) s_mask[t] = m;

@@ -966,6 +966,10 @@ JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *J
     if (pr.prim_screen) {
         a.prim_q = pr.d_prim_q.as<float>();
         a.prim_thr = fp32 ? PRIM_THRESHOLD32 : PRIM_THRESHOLD;
+        if (fp32) {  // whole shell quartets below the FP32 threshold are dropped before compaction
+            a.pairQ = pr.d_pairQ.as<float>();
+            a.ket_tau = PRIM_THRESHOLD32;
+        }
     }
     a.soa = fp32 ? pr.d_soa32.p : pr.d_soa64.p;
     a.soa_pairs = pr.d_soa_pairs.as<PairInfo>();

@@ -41,6 +41,9 @@ struct JKArgs {
     const float *Pm;         // [nshell][nshell] max |P| per shell block
     int nshell;
     float bound_lo, bound_hi;
+    // without density screening: a lane is evaluated only when Q_ab Q_cd >= ket_tau (0: all;
+    // the FP32 full builds cut at their primitive threshold)
+    float ket_tau;
     // primitive screening (accurate policy): the primitive quartet (ij|kl) is skipped when
     // q_ij q_kl <= prim_thr, q = sqrt(max_component (ij|ij)) of the primitive pair including its
     // contraction coefficients (a Cauchy-Schwarz bound on its contribution). Primitive pairs of
