@@ -404,6 +404,123 @@ hipError_t tc2_update(int64_t len, const double *grow, double *X, const double *
     return hipGetLastError();
 }
 
+// Batched symmetric eigensolver, one workgroup per matrix, the matrix in LDS. One-sided (Hestenes)
+// Jacobi on B = A + c I, with c from the Gershgorin bound so that every eigenvalue of B is at
+// least 1: the columns of U = B V converge to lambda_B,k v_k, so the eigenvalues of A are
+// ||u_k|| - c and its eigenvectors u_k / ||u_k|| (one n x n array, no separate V). Each step
+// rotates the n/2 disjoint column pairs of a round of the round-robin (circle) order, four lanes
+// per pair (n <= 128: all pairs of a round at once in 256 threads); a sweep
+// without a rotation ends the iteration (at most JACOBI_SWEEPS). Output as a dsyevd with vectors:
+// W ascending, the eigenvector of the k-th smallest eigenvalue at A[k * n + i].
+constexpr int JACOBI_SWEEPS = 40;
+constexpr double JACOBI_TOL = 1e-14;  // rotate while |u_p . u_q| > tol ||u_p|| ||u_q||
+__global__ __launch_bounds__(256) void jacobi_eig_kernel(const EigRec *__restrict__ recs) {
+    const EigRec r = recs[blockIdx.x];
+    const int n = r.n;
+    if (n <= 0 || n > JACOBI_MAX_N) return;  // the same for the whole workgroup
+    extern __shared__ double U[];  // [n][n]: column k at U[k * n]
+    __shared__ double s_red[4];
+    __shared__ double s_norm[JACOBI_MAX_N];
+    __shared__ int s_rank[JACOBI_MAX_N];
+    __shared__ int s_rot;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const double *__restrict__ A = r.A;
+    // Gershgorin lower bound of the symmetrised matrix
+    double lo = HUGE_VAL;
+    for (int i = tid; i < n; i += blockDim.x) {
+        double rs = 0.0;
+        for (int j = 0; j < n; ++j)
+            if (j != i) rs += fabs(0.5 * (A[i * n + j] + A[j * n + i]));
+        lo = fmin(lo, A[i * n + i] - rs);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lo = fmin(lo, __shfl_xor(lo, off, 64));
+    if (lane == 0) s_red[wave] = lo;
+    __syncthreads();
+    const double c = fmax(0.0, 1.0 - fmin(fmin(s_red[0], s_red[1]), fmin(s_red[2], s_red[3])));
+    for (int e = tid; e < n * n; e += blockDim.x) {
+        const int k = e / n, i = e - k * n;
+        U[e] = 0.5 * (A[i * n + k] + A[k * n + i]) + (i == k ? c : 0.0);
+    }
+    const int m = n + (n & 1);  // players of the round robin; index n is a bye when n is odd
+    for (int sweep = 0; sweep < JACOBI_SWEEPS; ++sweep) {
+        if (tid == 0) s_rot = 0;
+        __syncthreads();  // U loaded / previous sweep done; s_rot cleared
+        bool rotated = false;
+        for (int round = 0; round < m - 1; ++round) {
+            // circle method: position 0 fixed, positions 1..m-1 rotated by `round`; position j meets
+            // position m - 1 - j. Four lanes per pair (one quad), all pairs of the round at once.
+            const int j = tid >> 2, sub = tid & 3;
+            if (j < m / 2) {
+                const int a = j == 0 ? 0 : 1 + (j - 1 + round) % (m - 1);
+                const int b = 1 + (m - 2 - j + round) % (m - 1);
+                if (a < n && b < n) {  // not a bye (the same for the four lanes of the quad)
+                    const int p = min(a, b), q = max(a, b);
+                    double al = 0.0, be = 0.0, ga = 0.0;
+                    for (int i = sub; i < n; i += 4) {
+                        const double up = U[p * n + i], uq = U[q * n + i];
+                        al += up * up;
+                        be += uq * uq;
+                        ga += up * uq;
+                    }
+#pragma unroll
+                    for (int off = 1; off < 4; off <<= 1) {
+                        al += __shfl_xor(al, off, 4);
+                        be += __shfl_xor(be, off, 4);
+                        ga += __shfl_xor(ga, off, 4);
+                    }
+                    if (fabs(ga) > JACOBI_TOL * sqrt(al * be)) {  // the same for the quad
+                        const double zeta = (be - al) / (2.0 * ga);
+                        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                        for (int i = sub; i < n; i += 4) {
+                            const double up = U[p * n + i], uq = U[q * n + i];
+                            U[p * n + i] = cs * up - sn * uq;
+                            U[q * n + i] = sn * up + cs * uq;
+                        }
+                        rotated = true;
+                    }
+                }
+            }
+            __syncthreads();  // the next round's pairs share columns with this round's
+        }
+        if (rotated) s_rot = 1;  // any lane; every writer stores the same value
+        __syncthreads();
+        const bool again = s_rot != 0;
+        __syncthreads();  // every thread has read s_rot before the next sweep clears it
+        if (!again) break;
+    }
+    // eigenvalues ||u_k|| - c, ranks (ties by index), normalised vectors in rank order
+    for (int k = wave; k < n; k += 4) {
+        double s2 = 0.0;
+        for (int i = lane; i < n; i += 64) s2 += U[k * n + i] * U[k * n + i];
+        s2 = wave_sum(s2);
+        if (lane == 0) s_norm[k] = sqrt(s2);
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += blockDim.x) {
+        const double lk = s_norm[k];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const double lj = s_norm[j];
+            rank += (lj < lk || (lj == lk && j < k)) ? 1 : 0;
+        }
+        s_rank[k] = rank;
+        r.W[rank] = lk - c;
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += blockDim.x) {
+        const int k = e / n, i = e - k * n;
+        r.A[s_rank[k] * n + i] = U[e] / s_norm[k];
+    }
+}
+hipError_t jacobi_eig_batch(const EigRec *recs, int count, int max_n, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (max_n > JACOBI_MAX_N) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(jacobi_eig_kernel, dim3(count), dim3(256), (size_t)max_n * max_n * sizeof(double), s, recs);
+    return hipGetLastError();
+}
+
 __global__ void scatter_jk_batch_kernel(const ScatterRec *__restrict__ recs) {
     const ScatterRec r = recs[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;

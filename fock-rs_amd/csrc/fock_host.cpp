@@ -1646,9 +1646,35 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         HIPCHK(hipStreamSynchronize(st));  // bd goes out of scope
         HIPCHK(copy_f64((int64_t)nn, b.S.as<double>(), gA(b), st));
     }
+    // The eigenproblems of the batch: all molecules (or the unconverged ones) in one launch of the
+    // LDS-resident Jacobi solver when every basis fits it, else one rocSOLVER batched call per
+    // size group (a group whose molecules have all converged is skipped).
+    const bool use_jacobi = nmax <= JACOBI_MAX_N;
+    std::vector<EigRec> erec(count);
+    DevBuf d_erec;
+    HIPCHK(d_erec.ensure(count * sizeof(EigRec)));
+    auto eig_all = [&](bool only_active) -> fock_status {
+        if (use_jacobi) {
+            for (int m = 0; m < count; ++m) {
+                BatchMol &b = *mols[m];
+                erec[m] = EigRec{(only_active && b.done) ? 0 : b.n, gA(b), gW(b)};
+            }
+            HIPCHK(hipMemcpyAsync(d_erec.p, erec.data(), count * sizeof(EigRec), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));  // erec is rewritten by the next call
+            HIPCHK(jacobi_eig_batch(d_erec.as<EigRec>(), count, nmax, st));
+            return FOCK_OK;
+        }
+        for (auto &g : gr) {
+            bool any = !only_active;
+            for (int m : g->members) any = any || !mols[m]->done;
+            if (any)
+                STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
+                                g->work.as<double>(), g->info.as<int>()));
+        }
+        return FOCK_OK;
+    };
     // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146), eigenproblems batched
-    for (auto &g : gr) STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
-                                       g->work.as<double>(), g->info.as<int>()));
+    STCHK(eig_all(false));
     for (auto &bp : mols) {
         BatchMol &b = *bp;
         const int n = b.n;
@@ -1668,13 +1694,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             HIPCHK(gemm_f64(false, false, n, n, n, 1.0, fock_of(b), n, b.X.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
             HIPCHK(gemm_f64(true, false, n, n, n, 1.0, b.X.as<double>(), n, b.T1.as<double>(), n, 0.0, gA(b), n, st));
         }
-        for (auto &g : gr) {
-            bool any = false;  // a group whose molecules have all converged is skipped
-            for (int m : g->members) any = any || !mols[m]->done;
-            if (any)
-                STCHK(eig_batch(ctx, g->n, (int)g->members.size(), g->A.as<double>(), g->W.as<double>(),
-                                g->work.as<double>(), g->info.as<int>()));
-        }
+        STCHK(eig_all(true));
         for (auto &bp : mols) {
             BatchMol &b = *bp;
             if (b.done) continue;

@@ -131,6 +131,14 @@ hipError_t scatter_jk_batch(int nrec, const ScatterRec *recs, int max_npair, hip
 // J[f0(a) + c][f0(b) + d] += Jk[(c * nf(b) + d) * npair + j] for every pair j = (a, b), then Jk = 0
 hipError_t scatter_jk(int npair, const int2 *pair_shell, const int *shell_f0, const int *shell_nf, double *Jk,
                       double *J, int n, hipStream_t s);
+// one matrix of a batched symmetric eigenproblem (n = 0: skipped)
+struct EigRec {
+    int n;
+    double *A;  // in: symmetric n x n; out: eigenvector k (of the k-th smallest eigenvalue) at A[k * n]
+    double *W;  // out: eigenvalues, ascending
+};
+constexpr int JACOBI_MAX_N = 128;  // the matrix is held in LDS (128 KB at the limit)
+hipError_t jacobi_eig_batch(const EigRec *recs, int count, int max_n, hipStream_t s);
 // out = F + sigma (S - 0.5 SPS)
 hipError_t level_shift_fock(int64_t len, const double *F, const double *S, const double *SPS, double sigma,
                             double *out, hipStream_t s);
