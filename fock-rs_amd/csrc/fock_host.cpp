@@ -1480,6 +1480,28 @@ extern "C" fock_status fock_sorted_eigs(fock_ctx *ctx, int32_t n, const double *
     return FOCK_OK;
 }
 
+extern "C" fock_status fock_sorted_eigs_batch(fock_ctx *ctx, int32_t count, int32_t n, const double *M,
+                                              double *vectors, double *values) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (count <= 0 || n <= 0 || !M || !vectors || !values) return ctx->fail(FOCK_INVALID_ARG, "bad eigen arguments");
+    if (n > JACOBI_MAX_N) return ctx->fail(FOCK_UNSUPPORTED, "batched eigensolver: n above 128");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t nn = (size_t)n * n, bytes = (size_t)count * nn * sizeof(double);
+    DevBuf dA, dW, dR;
+    HIPCHK(dA.ensure(bytes));
+    HIPCHK(dW.ensure((size_t)count * n * sizeof(double)));
+    HIPCHK(dR.ensure((size_t)count * sizeof(EigRec)));
+    std::vector<EigRec> rec(count);
+    for (int i = 0; i < count; ++i) rec[i] = EigRec{n, dA.as<double>() + i * nn, dW.as<double>() + (size_t)i * n};
+    HIPCHK(hipMemcpyAsync(dA.p, M, bytes, hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(dR.p, rec.data(), count * sizeof(EigRec), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(jacobi_eig_batch(dR.as<EigRec>(), count, n, S(ctx)));
+    HIPCHK(hipMemcpyAsync(vectors, dA.p, bytes, hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipMemcpyAsync(values, dW.p, (size_t)count * n * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
 extern "C" fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const double *errors, const double *focks,
                                  double *out) {
     if (!ctx) return FOCK_INVALID_ARG;

@@ -47,7 +47,7 @@ EXPORTED = [
     "fock_scf_opts_default", "fock_scf_run", "fock_rhf_gradient", "fock_scf_batch_run", "fock_boys", "fock_eri_dense", "fock_one_electron",
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
     "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
-    "fock_jk_autotune", "fock_tiles_get", "fock_tiles_set",
+    "fock_jk_autotune", "fock_tiles_get", "fock_tiles_set", "fock_sorted_eigs_batch",
 ]
 
 
@@ -137,6 +137,7 @@ def load(path: Optional[str] = None):
         "fock_jk_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(JKStats)]),
         "fock_gemm": (ctypes.c_int, [vp, i32, i32, i32, i32, i32, d, _dp, i32, _dp, i32, d, _dp, i32]),
         "fock_sorted_eigs": (ctypes.c_int, [vp, i32, _dp, _dp, _dp]),
+        "fock_sorted_eigs_batch": (ctypes.c_int, [vp, i32, i32, _dp, _dp, _dp]),
         "fock_diis": (ctypes.c_int, [vp, i32, i64, _dp, _dp, _dp]),
         "fock_jk_set_timing": (ctypes.c_int, [vp, i32]),
         "fock_jk_get_timings": (ctypes.c_int, [vp, _dp]),
@@ -295,6 +296,17 @@ class Context:
         flat = np.ascontiguousarray(M.T.ravel())  # column-major storage
         self.check(self.lib.fock_sorted_eigs(self.ptr, n, _d(flat), _d(vec), _d(val)))
         return vec.reshape(n, n).T.copy(), val
+
+    def sorted_eigs_batch(self, Ms):
+        """Batched sorted_eigs of equal-size symmetric matrices (n <= 128): lists of (vectors as
+        columns, ascending values)."""
+        Ms = [np.asarray(M, dtype=np.float64) for M in Ms]
+        n = Ms[0].shape[0]
+        flat = np.ascontiguousarray(np.concatenate([M.T.ravel() for M in Ms]))  # column-major each
+        vec = np.zeros(len(Ms) * n * n)
+        val = np.zeros(len(Ms) * n)
+        self.check(self.lib.fock_sorted_eigs_batch(self.ptr, len(Ms), n, _d(flat), _d(vec), _d(val)))
+        return [(vec[i * n * n:(i + 1) * n * n].reshape(n, n).T.copy(), val[i * n:(i + 1) * n]) for i in range(len(Ms))]
 
     def diis(self, errors, focks):
         """scf::utils::diis; None when the DIIS matrix is singular."""

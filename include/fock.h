@@ -242,6 +242,13 @@ fock_status fock_gemm(fock_ctx *ctx, int32_t transA, int32_t transB, int32_t m, 
  * arrays (vectors column k = eigenvector k); values: n. Eigenvector signs are arbitrary. */
 fock_status fock_sorted_eigs(fock_ctx *ctx, int32_t n, const double *M, double *vectors, double *values);
 
+/* count eigen-decompositions of symmetric n x n matrices (n <= 128) in one launch, the batching
+ * form of fock_sorted_eigs (no reference counterpart; the solver of fock_scf_batch_run): one
+ * workgroup per matrix, one-sided Jacobi in LDS. M: count column-major matrices one after the
+ * other; vectors, values: as fock_sorted_eigs for each. */
+fock_status fock_sorted_eigs_batch(fock_ctx *ctx, int32_t count, int32_t n, const double *M, double *vectors,
+                                   double *values);
+
 /* DIIS extrapolation; replaces scf::utils::diis (scf/src/utils.rs:102-138): B_ij = <e_i, e_j>
  * (Frobenius) bordered by -1, solve B c = (0,...,0,-1), out = sum_i c_i F_i. errors and focks
  * hold m matrices of len doubles each (host). Returns FOCK_INVALID_ARG when B is singular (the

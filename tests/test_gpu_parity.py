@@ -254,6 +254,44 @@ def test_sorted_eigs(ctx):
     assert np.abs(v.T @ v - np.eye(64)).max() < 1e-12
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 13, 51, 104, 128])
+def test_sorted_eigs_batch_jacobi(ctx, n):
+    """Batched one-sided Jacobi (fock_sorted_eigs_batch) against LAPACK: random symmetric
+    matrices of different scales, a multiple of the identity and a spectrum with repeated values."""
+    rng = np.random.default_rng(n)
+    Ms = []
+    for scale in (1e-3, 1.0, 30.0):
+        M = rng.standard_normal((n, n)) * scale
+        Ms.append(M + M.T)
+    Ms.append(2.5 * np.eye(n))
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Ms.append(Q @ np.diag(np.repeat(np.arange((n + 2) // 3, dtype=float) - 4.0, 3)[:n]) @ Q.T)
+    for M, (v, w) in zip(Ms, ctx.sorted_eigs_batch(Ms)):
+        M = 0.5 * (M + M.T)
+        scale = max(1.0, np.abs(M).max()) * max(1, n) ** 0.5
+        assert np.all(np.diff(w) >= 0)
+        assert np.abs(w - np.linalg.eigvalsh(M)).max() < 1e-13 * scale * n
+        assert np.abs(M @ v - v * w).max() < 1e-13 * scale * n
+        assert np.abs(v.T @ v - np.eye(n)).max() < 1e-12
+
+
+def test_sorted_eigs_batch_fock_matrix_and_limits(ctx, o108):
+    """The converged C8H18/6-31G Fock matrix in the orthogonal basis (n = 108) against the
+    single-matrix rocSOLVER path; n above 128 is refused."""
+    S, T, V, J, K = (o108["o108_acc_" + k] for k in "STVJK")
+    _, U = np.linalg.eigh(S)
+    X = U @ np.diag(1 / np.sqrt(np.diag(U.T @ S @ U))) @ U.T
+    Fp = X.T @ (T + V + J - 0.5 * K) @ X
+    (v, w), = ctx.sorted_eigs_batch([Fp])
+    v1, w1 = ctx.sorted_eigs(Fp)
+    assert np.abs(w - w1).max() < 1e-11
+    P = v[:, :33] @ v[:, :33].T
+    P1 = v1[:, :33] @ v1[:, :33].T
+    assert np.abs(P - P1).max() < 1e-10  # the occupied space (the density) agrees
+    with pytest.raises(abi.FockError):
+        ctx.sorted_eigs_batch([np.eye(129)])
+
+
 def test_diis_matches_restatement(ctx):
     from oracle import refscf
     rng = np.random.default_rng(9)
