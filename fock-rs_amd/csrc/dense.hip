@@ -12,24 +12,41 @@ namespace fock {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
+// C = alpha op(A) op(B) + beta C on the FP64 matrix cores (v_mfma_f64_16x16x4): one workgroup per
+// 16 x 16 tile of C, the K range split over its four waves (wave w takes the k blocks 4w, 4w + 16,
+// ...), the partial tiles summed through LDS. Small tiles and the split keep the SCF's n <= 600
+// products spread over the CUs (n = 264: 289 workgroups) and each wave's chain of dependent
+// load -> MFMA steps a quarter as long.
 __global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool transB, int M, int N, int K,
                                                             double alpha, const double *__restrict__ A, int lda,
                                                             const double *__restrict__ B, int ldb, double beta,
                                                             double *__restrict__ C, int ldc) {
+    __shared__ double4_t part[3][64];  // partial tiles of waves 1..3
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile_m = blockIdx.y * 2 + (wave >> 1), tile_n = blockIdx.x * 2 + (wave & 1);
-    const int row0 = tile_m * 16, col0 = tile_n * 16;
-    if (row0 >= M || col0 >= N) return;  // wave-uniform
+    const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
     const int li = lane & 15, lk = lane >> 4;
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
     const int ar = row0 + li, bc = col0 + li;
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const int kk = k0 + lk;
-        double a = 0.0, b = 0.0;
-        if (ar < M && kk < K) a = transA ? A[(size_t)kk * lda + ar] : A[(size_t)ar * lda + kk];
-        if (bc < N && kk < K) b = transB ? B[(size_t)bc * ldb + kk] : B[(size_t)kk * ldb + bc];
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    auto load_a = [&](int kk) -> double {
+        return (ar < M && kk < K) ? (transA ? A[(size_t)kk * lda + ar] : A[(size_t)ar * lda + kk]) : 0.0;
+    };
+    auto load_b = [&](int kk) -> double {
+        return (bc < N && kk < K) ? (transB ? B[(size_t)bc * ldb + kk] : B[(size_t)kk * ldb + bc]) : 0.0;
+    };
+    // the operands of the next k block are loaded while the current one is multiplied
+    double a = load_a(4 * wave + lk), b = load_b(4 * wave + lk);
+    for (int k0 = 4 * wave; k0 < K; k0 += 16) {
+        const double an = load_a(k0 + 16 + lk), bn = load_b(k0 + 16 + lk);
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        a = an;
+        b = bn;
     }
+    if (wave > 0) part[wave - 1][lane] = acc;
+    __syncthreads();
+    if (wave != 0) return;
+    acc += part[0][lane];
+    acc += part[1][lane];
+    acc += part[2][lane];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = row0 + 4 * r + lk, j = col0 + li;
@@ -44,7 +61,7 @@ __global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool tr
 hipError_t gemm_f64(bool transA, bool transB, int m, int n, int k, double alpha, const double *A, int lda,
                     const double *B, int ldb, double beta, double *C, int ldc, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
-    dim3 grid((n + 31) / 32, (m + 31) / 32), block(256);
+    dim3 grid((n + 15) / 16, (m + 15) / 16), block(256);
     hipLaunchKernelGGL(gemm_f64_mfma_kernel, grid, block, 0, s, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta,
                        C, ldc);
     return hipGetLastError();

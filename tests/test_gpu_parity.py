@@ -231,7 +231,8 @@ def test_scf_o108_accurate_schwarz(ctx, o108):
 # ------------------------------------------------------------------ dense SCF steps
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("shape", [(16, 16, 4), (20, 13, 7), (108, 108, 33), (264, 264, 264)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (16, 16, 4), (20, 13, 7), (33, 17, 100), (108, 108, 33),
+                                   (264, 264, 264), (300, 40, 601)])
 def test_gemm_fp64_mfma(ctx, ta, tb, shape):
     mm, nn, kk = shape
     rng = np.random.default_rng(mm * 7 + nn)
