@@ -22,6 +22,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fock.h"
@@ -450,6 +451,23 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
 }
 
 namespace {
+// f(lo, hi) over [0, n) in up to 8 host threads (pure host work: no HIP calls inside f); small
+// ranges run on the calling thread.
+template <typename F>
+void parallel_for(int n, int grain, F f) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int T = std::min({8, hw, n / std::max(grain, 1)});
+    if (T <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (int t = 1; t < T; ++t) th.emplace_back([&, t]() { f((int)((int64_t)n * t / T), (int)((int64_t)n * (t + 1) / T)); });
+    f(0, (int)((int64_t)n / T));
+    for (auto &x : th) x.join();
+}
+
 // Blocks of class cls at a tile: runs of up to `tile` consecutive tasks of one bra pair (task
 // indices class-local), most expensive first (load balance; stable for determinism).
 void make_class_blocks(const Prepared &pr, int cls, int tile, std::vector<Block> &out) {
@@ -470,10 +488,14 @@ void make_class_blocks(const Prepared &pr, int cls, int tile, std::vector<Block>
 // Block lists of every class at the prepared tiles into d_blocks (room for tile 1 is kept, so
 // re-tiling never reallocates) and h_blocks.
 fock_status upload_blocks(fock_ctx *ctx, Prepared &pr) {
+    std::vector<Block> cb[NUM_CLASSES];
+    parallel_for(NUM_CLASSES, 1, [&](int lo, int hi) {
+        for (int cls = lo; cls < hi; ++cls) make_class_blocks(pr, cls, pr.tile[cls], cb[cls]);
+    });
     pr.h_blocks.clear();
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         pr.block_off[cls] = (int)pr.h_blocks.size();
-        make_class_blocks(pr, cls, pr.tile[cls], pr.h_blocks);
+        pr.h_blocks.insert(pr.h_blocks.end(), cb[cls].begin(), cb[cls].end());
     }
     pr.block_off[NUM_CLASSES] = (int)pr.h_blocks.size();
     HIPCHK(pr.d_blocks.ensure(std::max<size_t>(pr.h_tasks.size(), 1) * sizeof(Block)));
@@ -508,8 +530,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
 
     // every shell pair, oriented L(a) >= L(b) (same L: higher shell index first)
     std::vector<PairHost> pairs;
-    std::vector<PrimPair<double>> prims;
-    std::vector<double2> prim_ab;  // exponents of the two primitives of every primitive pair
+    pairs.reserve((size_t)ns * (ns + 1) / 2);
+    size_t nprim_all = 0;
     for (int s1 = 0; s1 < ns; ++s1)
         for (int s2 = 0; s2 <= s1; ++s2) {
             int a = s1, b = s2;
@@ -519,19 +541,29 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
             ph.sb = b;
             ph.cls = pair_class_of(pr.shells[a].L, pr.shells[b].L);
             ph.K = pr.shells[a].K * pr.shells[b].K;
-            ph.pp_start = (int)prims.size();
+            ph.pp_start = (int)nprim_all;
             ph.fa = pr.shells[a].f0;
             ph.fb = pr.shells[b].f0;
             ph.creation = (int)pairs.size();
             ph.dmax = 0.0;
-            for (int i = 0; i < pr.shells[a].K; ++i)
-                for (int j = 0; j < pr.shells[b].K; ++j) {
-                    prims.push_back(make_prim_pair(basis, pr.shells[a], pr.shells[b], i, j));
-                    prim_ab.push_back(make_double2(basis->exponents[pr.shells[a].prim0 + i],
-                                                   basis->exponents[pr.shells[b].prim0 + j]));
-                }
+            nprim_all += ph.K;
             pairs.push_back(ph);
         }
+    // primitive pairs of every shell pair (a-major), filled in parallel at their offsets
+    std::vector<PrimPair<double>> prims(nprim_all);
+    std::vector<double2> prim_ab(nprim_all);  // exponents of the two primitives of every primitive pair
+    parallel_for((int)pairs.size(), 256, [&](int lo, int hi) {
+        for (int t = lo; t < hi; ++t) {
+            const PairHost &ph = pairs[t];
+            const Shell &A = pr.shells[ph.sa], &B = pr.shells[ph.sb];
+            int e = ph.pp_start;
+            for (int i = 0; i < A.K; ++i)
+                for (int j = 0; j < B.K; ++j, ++e) {
+                    prims[e] = make_prim_pair(basis, A, B, i, j);
+                    prim_ab[e] = make_double2(basis->exponents[A.prim0 + i], basis->exponents[B.prim0 + j]);
+                }
+        }
+    });
     mark("pairs_prims");
     const int npair = (int)pairs.size();
     const size_t nprim = prims.size();
@@ -566,18 +598,21 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         std::vector<PrimPair<double>> sorted(nprim);
         std::vector<double2> sorted_ab(nprim);
         std::vector<float> qs(nprim);
-        std::vector<int> perm;
-        for (const PairHost &ph : pairs) {
-            perm.resize(ph.K);
-            std::iota(perm.begin(), perm.end(), 0);
-            std::stable_sort(perm.begin(), perm.end(),
-                             [&](int x, int y) { return q[ph.pp_start + x] > q[ph.pp_start + y]; });
-            for (int i = 0; i < ph.K; ++i) {
-                sorted[ph.pp_start + i] = prims[ph.pp_start + perm[i]];
-                sorted_ab[ph.pp_start + i] = prim_ab[ph.pp_start + perm[i]];
-                qs[ph.pp_start + i] = q[ph.pp_start + perm[i]];
+        parallel_for((int)pairs.size(), 256, [&](int lo, int hi) {
+            std::vector<int> perm;
+            for (int t = lo; t < hi; ++t) {
+                const PairHost &ph = pairs[t];
+                perm.resize(ph.K);
+                std::iota(perm.begin(), perm.end(), 0);
+                std::stable_sort(perm.begin(), perm.end(),
+                                 [&](int x, int y) { return q[ph.pp_start + x] > q[ph.pp_start + y]; });
+                for (int i = 0; i < ph.K; ++i) {
+                    sorted[ph.pp_start + i] = prims[ph.pp_start + perm[i]];
+                    sorted_ab[ph.pp_start + i] = prim_ab[ph.pp_start + perm[i]];
+                    qs[ph.pp_start + i] = q[ph.pp_start + perm[i]];
+                }
             }
-        }
+        });
         prims.swap(sorted);
         prim_ab.swap(sorted_ab);
         prim_q_host = qs;
@@ -740,62 +775,79 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     st.n_basis = n;
     st.n_shells = ns;
     st.n_shell_pairs = npair;
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
-        pr.task_off[cls] = (int)tasks.size();
-        const int X = CLASS_BRA[cls], Y = CLASS_KET[cls];
+    // the task lists of the six classes are independent: built in parallel, joined in class order
+    struct ClassTasks {
         std::vector<Task> ct;
-        std::vector<int> cgid;
-        int ngroup = 0;
-        for (int i = 0; i < npair; ++i) {
-            if (pr.pairs[i].cls != X) continue;
-            const double di = pr.pairs[i].dmax;
-            for (const Bucket &bk : buckets) {
-                if (bk.cls != Y) continue;
-                // hi: kets passing the screen form a prefix (dmax descending within the bucket)
-                int lo = bk.begin, hi = bk.end;
-                {
-                    int l = bk.begin, h = bk.end;
-                    while (l < h) {
-                        int m = (l + h) / 2;
-                        if (keep(di, pr.pairs[m].dmax)) l = m + 1;
-                        else h = m;
+        std::vector<int> gf, gl;
+        int64_t quartets = 0, prim_quartets = 0;
+    } per[NUM_CLASSES];
+    parallel_for(NUM_CLASSES, 1, [&](int clo, int chi) {
+        for (int cls = clo; cls < chi; ++cls) {
+            ClassTasks &out = per[cls];
+            const int X = CLASS_BRA[cls], Y = CLASS_KET[cls];
+            std::vector<Task> &ct = out.ct;
+            std::vector<int> cgid;
+            int ngroup = 0;
+            for (int i = 0; i < npair; ++i) {
+                if (pr.pairs[i].cls != X) continue;
+                const double di = pr.pairs[i].dmax;
+                for (const Bucket &bk : buckets) {
+                    if (bk.cls != Y) continue;
+                    // hi: kets passing the screen form a prefix (dmax descending within the bucket)
+                    int lo = bk.begin, hi = bk.end;
+                    {
+                        int l = bk.begin, h = bk.end;
+                        while (l < h) {
+                            int m = (l + h) / 2;
+                            if (keep(di, pr.pairs[m].dmax)) l = m + 1;
+                            else h = m;
+                        }
+                        hi = l;
                     }
-                    hi = l;
-                }
-                if (screen == FOCK_SCREEN_REF_BF_PAIR && X == Y && pr.pairs[i].K == bk.K && hi <= i && i < bk.end) {
-                    // the pair with itself holds the diagonal integrals the reference never screens
-                    hi = i + 1;
-                }
-                if (X == Y) {  // unique quartets: ket rank >= bra rank
-                    int l = bk.begin, h = hi;
-                    while (l < h) {
-                        int m = (l + h) / 2;
-                        if (rank_sorted[m] < rank_sorted[i]) l = m + 1;
-                        else h = m;
+                    if (screen == FOCK_SCREEN_REF_BF_PAIR && X == Y && pr.pairs[i].K == bk.K && hi <= i &&
+                        i < bk.end) {
+                        // the pair with itself holds the diagonal integrals the reference never screens
+                        hi = i + 1;
                     }
-                    lo = l;
+                    if (X == Y) {  // unique quartets: ket rank >= bra rank
+                        int l = bk.begin, h = hi;
+                        while (l < h) {
+                            int m = (l + h) / 2;
+                            if (rank_sorted[m] < rank_sorted[i]) l = m + 1;
+                            else h = m;
+                        }
+                        lo = l;
+                    }
+                    for (int s = lo; s < hi; s += 64) {
+                        const int cnt = std::min(64, hi - s);
+                        ct.push_back(Task{i, s, cnt, pr.pairs[i].K * bk.K});
+                        cgid.push_back(ngroup);
+                        out.quartets += cnt;
+                        out.prim_quartets += (int64_t)cnt * pr.pairs[i].K * bk.K;
+                    }
+                    if (hi > lo) ++ngroup;
                 }
-                for (int s = lo; s < hi; s += 64) {
-                    const int cnt = std::min(64, hi - s);
-                    ct.push_back(Task{i, s, cnt, pr.pairs[i].K * bk.K});
-                    cgid.push_back(ngroup);
-                    st.shell_quartets[cls] += cnt;
-                    st.primitive_quartets[cls] += (int64_t)cnt * pr.pairs[i].K * bk.K;
-                }
-                if (hi > lo) ++ngroup;
             }
+            // tasks stay bra-major (the tasks of one bra pair, and of one (bra, bucket) group, are
+            // adjacent); blocks of up to a tile of tasks of one bra pair feed the blocked digestion
+            out.gf.resize(ct.size());
+            out.gl.resize(ct.size());
+            for (size_t q = 0; q < ct.size(); ++q)
+                out.gf[q] = (q > 0 && cgid[q] == cgid[q - 1]) ? out.gf[q - 1] : (int)q;
+            for (size_t q = ct.size(); q-- > 0;)
+                out.gl[q] = (q + 1 < ct.size() && cgid[q] == cgid[q + 1]) ? out.gl[q + 1] : (int)q;
         }
-        // tasks stay bra-major (the tasks of one bra pair, and of one (bra, bucket) group, are
-        // adjacent); blocks of up to a tile of tasks of one bra pair feed the blocked digestion
-        const std::vector<int> &sg = cgid;
-        std::vector<int> gf(ct.size()), gl(ct.size());
-        for (size_t q = 0; q < ct.size(); ++q) gf[q] = (q > 0 && sg[q] == sg[q - 1]) ? gf[q - 1] : (int)q;
-        for (size_t q = ct.size(); q-- > 0;) gl[q] = (q + 1 < ct.size() && sg[q] == sg[q + 1]) ? gl[q + 1] : (int)q;
-        gfirst_all.insert(gfirst_all.end(), gf.begin(), gf.end());
-        glast_all.insert(glast_all.end(), gl.begin(), gl.end());
-        pr.max_slots = std::max<int64_t>(pr.max_slots, (int64_t)ct.size() * 64);
-        tasks.insert(tasks.end(), ct.begin(), ct.end());
-        st.tasks[cls] = (int64_t)ct.size();
+    });
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        const ClassTasks &c = per[cls];
+        pr.task_off[cls] = (int)tasks.size();
+        gfirst_all.insert(gfirst_all.end(), c.gf.begin(), c.gf.end());
+        glast_all.insert(glast_all.end(), c.gl.begin(), c.gl.end());
+        pr.max_slots = std::max<int64_t>(pr.max_slots, (int64_t)c.ct.size() * 64);
+        tasks.insert(tasks.end(), c.ct.begin(), c.ct.end());
+        st.shell_quartets[cls] = c.quartets;
+        st.primitive_quartets[cls] = c.prim_quartets;
+        st.tasks[cls] = (int64_t)c.ct.size();
         st.integrals[cls] = st.shell_quartets[cls] * CLASS_NINT[cls];
         st.algorithmic_flop[cls] = (double)st.primitive_quartets[cls] * F_PRIM[cls] + 12.0 * (double)st.integrals[cls];
     }
@@ -2492,8 +2544,10 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         }
         phase_mark("diis");
         // F' = X^T F X, eigen, C = X C', P_new (:210-216); the orbitals are computed whenever this
-        // iteration may be the returned one
-        const bool may_exit = err < opts.epsilon || iter >= opts.max_iters - 1 || !is_normal(err);
+        // iteration may be the returned one (never a bulk-phase or incremental build: those hand
+        // over to an FP64 full build before returning)
+        const bool returnable = phase == 1 && !incr_build;
+        const bool may_exit = (returnable && err < opts.epsilon) || iter >= opts.max_iters - 1 || !is_normal(err);
         STCHK(diagonalize(Fuse, may_exit));
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
         HIPCHK(energy_trace(n, P_, H_, G_, scal + 1, S(ctx)));
