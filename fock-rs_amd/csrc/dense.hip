@@ -474,6 +474,7 @@ __global__ __launch_bounds__(256) void jacobi_eig_kernel(const EigRec *__restric
                 if (a < n && b < n) {  // not a bye (the same for the four lanes of the quad)
                     const int p = min(a, b), q = max(a, b);
                     double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll 4
                     for (int i = sub; i < n; i += 4) {
                         const double up = U[p * n + i], uq = U[q * n + i];
                         al += up * up;
@@ -490,6 +491,7 @@ __global__ __launch_bounds__(256) void jacobi_eig_kernel(const EigRec *__restric
                         const double zeta = (be - al) / (2.0 * ga);
                         const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                         const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll 4
                         for (int i = sub; i < n; i += 4) {
                             const double up = U[p * n + i], uq = U[q * n + i];
                             U[p * n + i] = cs * up - sn * uq;

@@ -365,6 +365,40 @@ __global__ __launch_bounds__(256) void eri_diag_kernel(DiagArgs args) {
         }
 }
 
+// The same diagonal with one wave per pair: the lanes take the K^2 primitive quartets in turn and
+// the diagonal components are summed over the wave. Accurate policy only; the reference-compatible
+// policy keeps the serial order of eri_diag_kernel (its 1e-6 pair screen compares these values
+// against a threshold).
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void eri_diag_wave_kernel(DiagArgs args) {
+    constexpr int NA = ncart(LA), NB = ncart(LB), NAB = NA * NB;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= args.npairs) return;  // wave-uniform
+    const PairInfo pr = args.pairs[args.first + t];
+    const PrimPair<double> *__restrict__ pp = args.prims + pr.pp_start;
+    double I[NAB][NAB];
+#pragma unroll
+    for (int x = 0; x < NAB; ++x)
+#pragma unroll
+        for (int y = 0; y < NAB; ++y) I[x][y] = 0.0;
+    const int KK = pr.K * pr.K;
+    for (int e = lane; e < KK; e += 64) {
+        const int i = e / pr.K, k = e - i * pr.K;
+        accumulate_prim_quartet<LA, LB, LA, LB, 1, double>(pp[i], pp[k], args.bt, I);
+    }
+    const size_t N = (size_t)args.nbf;
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const double v = wave_sum(I[a * NB + b][a * NB + b]);
+            if (lane == 0) {
+                args.D[(pr.fa + a) * N + (pr.fb + b)] = v;
+                args.D[(pr.fb + b) * N + (pr.fa + a)] = v;
+            }
+        }
+}
+
 // q = sqrt(max_component (ij|ij)) of single primitive pairs (contraction coefficients
 // included), the per-primitive Cauchy-Schwarz factor of the primitive screen.
 template <int LA, int LB>
@@ -769,14 +803,21 @@ hipError_t launch_prim_count(const PrimCountArgs &a, hipStream_t s) {
 
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
-    dim3 grid((a.npairs + 255) / 256), block(256);
-    switch (pair_class * 2 + (policy ? 1 : 0)) {
+    if (policy) {  // accurate: one wave per pair
+        dim3 wgrid((a.npairs + 3) / 4), wblock(256);
+        switch (pair_class) {
+            case 0: hipLaunchKernelGGL((eri_diag_wave_kernel<0, 0>), wgrid, wblock, 0, s, a); break;
+            case 1: hipLaunchKernelGGL((eri_diag_wave_kernel<1, 0>), wgrid, wblock, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((eri_diag_wave_kernel<1, 1>), wgrid, wblock, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    dim3 grid((a.npairs + 255) / 256), block(256);  // reference-compatible: one thread per pair
+    switch (pair_class) {
         case 0: hipLaunchKernelGGL((eri_diag_kernel<0, 0, 0>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((eri_diag_kernel<0, 0, 1>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((eri_diag_kernel<1, 0, 0>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((eri_diag_kernel<1, 0, 1>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((eri_diag_kernel<1, 1, 0>), grid, block, 0, s, a); break;
-        case 5: hipLaunchKernelGGL((eri_diag_kernel<1, 1, 1>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((eri_diag_kernel<1, 0, 0>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((eri_diag_kernel<1, 1, 0>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
