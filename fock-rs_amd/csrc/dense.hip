@@ -17,13 +17,12 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // ...), the partial tiles summed through LDS. Small tiles and the split keep the SCF's n <= 600
 // products spread over the CUs (n = 264: 289 workgroups) and each wave's chain of dependent
 // load -> MFMA steps a quarter as long.
-__global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool transB, int M, int N, int K,
-                                                            double alpha, const double *__restrict__ A, int lda,
-                                                            const double *__restrict__ B, int ldb, double beta,
-                                                            double *__restrict__ C, int ldc) {
+__device__ __forceinline__ void gemm_f64_tile(bool transA, bool transB, int M, int N, int K, double alpha,
+                                              const double *__restrict__ A, int lda, const double *__restrict__ B,
+                                              int ldb, double beta, double *__restrict__ C, int ldc, int row0,
+                                              int col0) {
     __shared__ double4_t part[3][64];  // partial tiles of waves 1..3
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
     const int li = lane & 15, lk = lane >> 4;
     const int ar = row0 + li, bc = col0 + li;
     double4_t acc = {0.0, 0.0, 0.0, 0.0};
@@ -56,6 +55,35 @@ __global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool tr
             C[(size_t)i * ldc + j] = v;
         }
     }
+}
+
+// C = alpha op(A) op(B) + beta C on the FP64 matrix cores (v_mfma_f64_16x16x4): one workgroup per
+// 16 x 16 tile of C, the K range split over its four waves (wave w takes the k blocks 4w, 4w + 16,
+// ...), the partial tiles summed through LDS. Small tiles and the split keep the SCF's n <= 600
+// products spread over the CUs (n = 264: 289 workgroups) and each wave's chain of dependent
+// load -> MFMA steps a quarter as long.
+__global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool transB, int M, int N, int K,
+                                                            double alpha, const double *__restrict__ A, int lda,
+                                                            const double *__restrict__ B, int ldb, double beta,
+                                                            double *__restrict__ C, int ldc) {
+    gemm_f64_tile(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.y * 16, blockIdx.x * 16);
+}
+
+// the same over a batch: blockIdx.z is the record; tiles outside a record's C return at once
+__global__ __launch_bounds__(256) void gemm_f64_batch_kernel(bool transA, bool transB, double alpha, double beta,
+                                                             const GemmRec *__restrict__ recs) {
+    const GemmRec r = recs[blockIdx.z];
+    const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
+    if (row0 >= r.m || col0 >= r.n) return;  // the same for the whole workgroup
+    gemm_f64_tile(transA, transB, r.m, r.n, r.k, alpha, r.A, r.lda, r.B, r.ldb, beta, r.C, r.ldc, row0, col0);
+}
+
+hipError_t gemm_f64_batch(bool transA, bool transB, double alpha, double beta, const GemmRec *recs, int count,
+                          int max_m, int max_n, hipStream_t s) {
+    if (count <= 0 || max_m <= 0 || max_n <= 0) return hipSuccess;
+    dim3 grid((max_n + 15) / 16, (max_m + 15) / 16, count), block(256);
+    hipLaunchKernelGGL(gemm_f64_batch_kernel, grid, block, 0, s, transA, transB, alpha, beta, recs);
+    return hipGetLastError();
 }
 
 hipError_t gemm_f64(bool transA, bool transB, int m, int n, int k, double alpha, const double *A, int lda,
@@ -184,9 +212,9 @@ hipError_t density_from_coeffs(int n, int nocc, const double *C, double *P, hipS
     return gemm_f64(false, true, n, n, nocc, 2.0, C, n, C, n, 0.0, P, n, s);
 }
 
-__global__ __launch_bounds__(256) void energy_trace_kernel(int n, const double *__restrict__ P,
-                                                           const double *__restrict__ H,
-                                                           const double *__restrict__ G, double *__restrict__ out) {
+__device__ __forceinline__ void energy_trace_kernel_body(int n, const double *__restrict__ P,
+                                                         const double *__restrict__ H, const double *__restrict__ G,
+                                                         double *__restrict__ out) {
     __shared__ double red[256];
     const int64_t len = (int64_t)n * n;
     double acc = 0.0;
@@ -198,6 +226,11 @@ __global__ __launch_bounds__(256) void energy_trace_kernel(int n, const double *
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = 0.5 * red[0];
+}
+__global__ __launch_bounds__(256) void energy_trace_kernel(int n, const double *__restrict__ P,
+                                                           const double *__restrict__ H,
+                                                           const double *__restrict__ G, double *__restrict__ out) {
+    energy_trace_kernel_body(n, P, H, G, out);
 }
 
 hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *out, hipStream_t s) {
@@ -575,6 +608,98 @@ hipError_t level_shift_fock(int64_t len, const double *F, const double *Sm, cons
                             double *out, hipStream_t s) {
     hipLaunchKernelGGL(level_shift_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, F, Sm, SPS,
                        sigma, out);
+    return hipGetLastError();
+}
+
+// DIIS error of every molecule (blockIdx.y): e = FPS - (FPS)^T with max |e| (NaN kept), and the
+// copy of F into its ring slot
+__global__ __launch_bounds__(256) void diis_error_batch_kernel(const DiisRec *__restrict__ recs) {
+    __shared__ double red[256];
+    const DiisRec r = recs[blockIdx.y];
+    const int64_t len = (int64_t)r.n * r.n;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)blockIdx.x * blockDim.x >= len) return;  // the same for the whole workgroup
+    double v = 0.0;
+    if (idx < len) {
+        const int i = (int)(idx / r.n), j = (int)(idx % r.n);
+        const double d = r.FPS[idx] - r.FPS[(int64_t)j * r.n + i];
+        r.e[idx] = d;
+        r.f_slot[idx] = r.F[idx];
+        v = fabs(d);
+        if (d != d) v = d;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const double o = red[threadIdx.x + off];
+            const double m = red[threadIdx.x];
+            red[threadIdx.x] = (o != o || o > m) ? o : m;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomic_max_nonneg(r.dots + 16, red[0]);
+}
+// <e, errs[s]> for the 16 ring slots of every molecule: one workgroup per (slot, molecule), a fixed
+// reduction order
+__global__ __launch_bounds__(256) void diis_dots_batch_kernel(const DiisRec *__restrict__ recs) {
+    __shared__ double red[256];
+    const DiisRec r = recs[blockIdx.y];
+    const int64_t len = (int64_t)r.n * r.n;
+    const double *b = r.errs + (int64_t)blockIdx.x * len;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < len; i += blockDim.x) acc += r.e[i] * b[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) r.dots[blockIdx.x] = red[0];
+}
+hipError_t diis_error_batch(const DiisRec *recs, int count, int max_n, hipStream_t s) {
+    if (count <= 0 || max_n <= 0) return hipSuccess;
+    const int64_t len = (int64_t)max_n * max_n;
+    hipLaunchKernelGGL(diis_error_batch_kernel, dim3((unsigned)((len + 255) / 256), count), dim3(256), 0, s, recs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(diis_dots_batch_kernel, dim3(16, count), dim3(256), 0, s, recs);
+    return hipGetLastError();
+}
+
+__global__ void lincomb_batch_kernel(const CombRec *__restrict__ recs) {
+    const CombRec r = recs[blockIdx.y];
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= r.len) return;
+    double acc = r.c[0] * r.base[(int64_t)r.slots[0] * r.len + x];
+    for (int i = 1; i < r.count; ++i) acc = acc + r.c[i] * r.base[(int64_t)r.slots[i] * r.len + x];
+    r.out[x] = acc;
+}
+hipError_t lincomb_batch(const CombRec *recs, int count, int64_t max_len, hipStream_t s) {
+    if (count <= 0 || max_len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(lincomb_batch_kernel, dim3((unsigned)((max_len + 255) / 256), count), dim3(256), 0, s, recs);
+    return hipGetLastError();
+}
+
+__global__ void level_shift_batch_kernel(const ShiftRec *__restrict__ recs, double sigma) {
+    const ShiftRec r = recs[blockIdx.y];
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < r.len) r.F[x] = r.F[x] + sigma * (r.S[x] - 0.5 * r.SPS[x]);
+}
+hipError_t level_shift_batch(const ShiftRec *recs, int count, int64_t max_len, double sigma, hipStream_t s) {
+    if (count <= 0 || max_len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(level_shift_batch_kernel, dim3((unsigned)((max_len + 255) / 256), count), dim3(256), 0, s, recs,
+                       sigma);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void energy_trace_batch_kernel(const TraceRec *__restrict__ recs) {
+    const TraceRec r = recs[blockIdx.x];
+    energy_trace_kernel_body(r.n, r.P, r.H, r.G, r.out);
+}
+hipError_t energy_trace_batch(const TraceRec *recs, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(energy_trace_batch_kernel, dim3(count), dim3(256), 0, s, recs);
     return hipGetLastError();
 }
 

@@ -1597,7 +1597,7 @@ struct BatchMol {
     Prepared pr;
     int n = 0, nocc = 0, group = 0, gidx = 0;
     double e_nuc = 0.0;
-    DevBuf S, T, V, H, X, U, T1, T2, P, G, F, Fx, C, errs, focks, dots, coef, slots;
+    DevBuf S, T, V, H, X, U, T1, T2, P, G, F, Fx, C, errs, focks;
     std::deque<int> ring;                 // stored (error, Fock) slots, oldest first
     std::deque<std::vector<double>> dotm;  // <e_i, e_j> of the stored slots
     int next_slot = 0;
@@ -1708,9 +1708,6 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             HIPCHK(d->ensure(bytes));
         HIPCHK(b.errs.ensure(16 * bytes));
         HIPCHK(b.focks.ensure(16 * bytes));
-        HIPCHK(b.dots.ensure(17 * sizeof(double)));
-        HIPCHK(b.coef.ensure(16 * sizeof(double)));
-        HIPCHK(b.slots.ensure(16 * sizeof(int)));
         HIPCHK(fill_f64(16 * (int64_t)nn, 0.0, b.errs.as<double>(), st));
         BasisDev bd;
         STCHK(upload_one_electron(ctx, b.basis, b.nuc, b.pr.shells, bd));
@@ -1720,22 +1717,55 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         HIPCHK(hipStreamSynchronize(st));  // bd goes out of scope
         HIPCHK(copy_f64((int64_t)nn, b.S.as<double>(), gA(b), st));
     }
+    // Small per-launch tables (records of the batched steps, DIIS coefficients) go through a pinned
+    // host buffer mirrored on the device, bump-allocated and reset only after a synchronisation of
+    // the stream (when every copy out of it has completed).
+    struct Staging {
+        char *h = nullptr;
+        ~Staging() {
+            if (h) (void)hipHostFree(h);
+        }
+    } stg;
+    constexpr size_t STG_CAP = (size_t)1 << 21;
+    DevBuf d_stg;
+    size_t stg_off = 0;
+    HIPCHK(hipHostMalloc((void **)&stg.h, STG_CAP, hipHostMallocDefault));
+    HIPCHK(d_stg.ensure(STG_CAP));
+    auto stage = [&](const void *src, size_t bytes, const void **dev) -> fock_status {
+        const size_t at = (stg_off + 63) & ~(size_t)63;
+        if (at + bytes > STG_CAP) return ctx->fail(FOCK_OOM, "batch staging buffer full");
+        std::memcpy(stg.h + at, src, bytes);
+        HIPCHK(hipMemcpyAsync(d_stg.as<char>() + at, stg.h + at, bytes, hipMemcpyHostToDevice, st));
+        *dev = d_stg.as<char>() + at;
+        stg_off = at + bytes;
+        return FOCK_OK;
+    };
+    auto gemm_all = [&](bool ta, bool tb, double alpha, const std::vector<GemmRec> &r) -> fock_status {
+        if (r.empty()) return FOCK_OK;
+        int mm = 0, mn = 0;
+        for (const GemmRec &g : r) {
+            mm = std::max(mm, g.m);
+            mn = std::max(mn, g.n);
+        }
+        const void *d = nullptr;
+        STCHK(stage(r.data(), r.size() * sizeof(GemmRec), &d));
+        HIPCHK(gemm_f64_batch(ta, tb, alpha, 0.0, static_cast<const GemmRec *>(d), (int)r.size(), mm, mn, st));
+        return FOCK_OK;
+    };
     // The eigenproblems of the batch: all molecules (or the unconverged ones) in one launch of the
     // LDS-resident Jacobi solver when every basis fits it, else one rocSOLVER batched call per
     // size group (a group whose molecules have all converged is skipped).
     const bool use_jacobi = nmax <= JACOBI_MAX_N;
-    std::vector<EigRec> erec(count);
-    DevBuf d_erec;
-    HIPCHK(d_erec.ensure(count * sizeof(EigRec)));
     auto eig_all = [&](bool only_active) -> fock_status {
         if (use_jacobi) {
+            std::vector<EigRec> erec(count);
             for (int m = 0; m < count; ++m) {
                 BatchMol &b = *mols[m];
                 erec[m] = EigRec{(only_active && b.done) ? 0 : b.n, gA(b), gW(b)};
             }
-            HIPCHK(hipMemcpyAsync(d_erec.p, erec.data(), count * sizeof(EigRec), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));  // erec is rewritten by the next call
-            HIPCHK(jacobi_eig_batch(d_erec.as<EigRec>(), count, nmax, st));
+            const void *d = nullptr;
+            STCHK(stage(erec.data(), erec.size() * sizeof(EigRec), &d));
+            HIPCHK(jacobi_eig_batch(static_cast<const EigRec *>(d), count, nmax, st));
             return FOCK_OK;
         }
         for (auto &g : gr) {
@@ -1759,23 +1789,24 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         HIPCHK(scale_columns(n, U, T1, n, T2, st));
         HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U, n, 0.0, b.X.as<double>(), n, st));
     }
-    // F' = X^T M X for every molecule into its group slot, batched eigensolve, C = X R'^T, P
+    // F' = X^T M X for every molecule into its group slot, batched eigensolve, C = X R'^T,
+    // P = 2 C_occ C_occ^T: one launch per step for all unconverged molecules
     auto diagonalize_all = [&](auto fock_of) -> fock_status {
+        std::vector<GemmRec> g1, g2, g3, g4;
         for (auto &bp : mols) {
             BatchMol &b = *bp;
             if (b.done) continue;
             const int n = b.n;
-            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, fock_of(b), n, b.X.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
-            HIPCHK(gemm_f64(true, false, n, n, n, 1.0, b.X.as<double>(), n, b.T1.as<double>(), n, 0.0, gA(b), n, st));
+            g1.push_back(GemmRec{n, n, n, n, n, n, fock_of(b), b.X.as<double>(), b.T1.as<double>()});
+            g2.push_back(GemmRec{n, n, n, n, n, n, b.X.as<double>(), b.T1.as<double>(), gA(b)});
+            g3.push_back(GemmRec{n, n, n, n, n, n, b.X.as<double>(), gA(b), b.C.as<double>()});
+            g4.push_back(GemmRec{n, n, b.nocc, n, n, n, b.C.as<double>(), b.C.as<double>(), b.P.as<double>()});
         }
+        STCHK(gemm_all(false, false, 1.0, g1));  // F X
+        STCHK(gemm_all(true, false, 1.0, g2));   // X^T (F X)
         STCHK(eig_all(true));
-        for (auto &bp : mols) {
-            BatchMol &b = *bp;
-            if (b.done) continue;
-            const int n = b.n;
-            HIPCHK(gemm_f64(false, true, n, n, n, 1.0, b.X.as<double>(), n, gA(b), n, 0.0, b.C.as<double>(), n, st));
-            HIPCHK(density_from_coeffs(n, b.nocc, b.C.as<double>(), b.P.as<double>(), st));
-        }
+        STCHK(gemm_all(false, true, 1.0, g3));   // C = X R'^T
+        STCHK(gemm_all(false, true, 2.0, g4));   // P = 2 C_occ C_occ^T (scf/src/lib.rs:214-216)
         return FOCK_OK;
     };
     phase_mark("one_elec_X");
@@ -1999,28 +2030,51 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         }
         (void)hipGetLastError();  // a failed capture leaves no launch behind; fall back to eager
     }
+    // per molecule m: dots_all[18 m + s] = <e, e_s> (s < 16), [18 m + 16] = max |e|, [18 m + 17] = E_elec
+    DevBuf d_dots;
+    HIPCHK(d_dots.ensure((size_t)count * 18 * sizeof(double)));
+    double *dots_all = d_dots.as<double>();
+    HIPCHK(hipStreamSynchronize(st));
     for (int iter = 0; iter <= opts.max_iters && active > 0; ++iter) {
+        stg_off = 0;  // the previous iteration ended with a synchronisation
         if (graph.x) HIPCHK(hipGraphLaunch(graph.x, st));
         else STCHK(enqueue_build());
         phase_mark("fock_build");
-        for (auto &bp : mols) {
-            BatchMol &b = *bp;
-            if (b.done) continue;
-            const int n = b.n;
-            const int64_t nn = (int64_t)n * n;
-            // DIIS error FPS - SPF (scf/src/lib.rs:189-192) into the ring, dots with every slot
-            const int slot = b.next_slot;
-            double *e_slot = b.errs.as<double>() + (int64_t)slot * nn;
-            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.F.as<double>(), n, b.P.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
-            HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.T1.as<double>(), n, b.S.as<double>(), n, 0.0, b.T2.as<double>(), n, st));
-            HIPCHK(fill_f64(1, 0.0, b.dots.as<double>() + 16, st));
-            HIPCHK(antisym_maxabs(n, b.T2.as<double>(), e_slot, b.dots.as<double>() + 16, st));
-            HIPCHK(copy_f64(nn, b.F.as<double>(), b.focks.as<double>() + (int64_t)slot * nn, st));
-            HIPCHK(frob_dots(nn, e_slot, b.errs.as<double>(), nn, 16, b.dots.as<double>(), st));
-            HIPCHK(hipMemcpyAsync(hostd.data() + (size_t)(&bp - &mols[0]) * 18, b.dots.p, 17 * sizeof(double),
-                                  hipMemcpyDeviceToHost, st));
+        // DIIS error FPS - SPF (scf/src/lib.rs:189-192) into the ring, dots with every slot
+        {
+            std::vector<GemmRec> g1, g2;
+            std::vector<DiisRec> dr;
+            int dmax_n = 0;
+            for (int m = 0; m < count; ++m) {
+                BatchMol &b = *mols[m];
+                if (b.done) continue;
+                const int n = b.n;
+                const int64_t nn = (int64_t)n * n;
+                const int slot = b.next_slot;
+                g1.push_back(GemmRec{n, n, n, n, n, n, b.F.as<double>(), b.P.as<double>(), b.T1.as<double>()});
+                g2.push_back(GemmRec{n, n, n, n, n, n, b.T1.as<double>(), b.S.as<double>(), b.T2.as<double>()});
+                dr.push_back(DiisRec{n, b.T2.as<double>(), b.errs.as<double>() + (int64_t)slot * nn, b.F.as<double>(),
+                                     b.focks.as<double>() + (int64_t)slot * nn, b.errs.as<double>(),
+                                     dots_all + (size_t)m * 18});
+                dmax_n = std::max(dmax_n, n);
+            }
+            HIPCHK(fill_f64((int64_t)count * 18, 0.0, dots_all, st));
+            STCHK(gemm_all(false, false, 1.0, g1));  // F P
+            STCHK(gemm_all(false, false, 1.0, g2));  // (F P) S
+            const void *d = nullptr;
+            STCHK(stage(dr.data(), dr.size() * sizeof(DiisRec), &d));
+            HIPCHK(diis_error_batch(static_cast<const DiisRec *>(d), (int)dr.size(), dmax_n, st));
+            HIPCHK(hipMemcpyAsync(hostd.data(), dots_all, (size_t)count * 18 * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+            HIPCHK(hipStreamSynchronize(st));
         }
-        HIPCHK(hipStreamSynchronize(st));
+        std::vector<CombRec> comb;
+        std::vector<GemmRec> ls1, ls2;
+        std::vector<ShiftRec> shift;
+        std::vector<double> coef_all;  // every molecule's coefficients and slots, staged at once
+        std::vector<int> slot_all;
+        std::vector<size_t> comb_at;
+        int64_t comb_len = 0;
         for (int m = 0; m < count; ++m) {
             BatchMol &b = *mols[m];
             if (b.done) continue;
@@ -2041,41 +2095,71 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             for (int i = 0; i < mm - 1; ++i) b.dotm[i].push_back(nd[i]);
             b.dotm.push_back(nd);
             b.err = hd[16];
-            const double *Fuse = b.focks.as<double>() + (int64_t)slot * nn;
+            // Fx = the DIIS extrapolation (:202-207) or the newest Fock matrix, as one linear
+            // combination of ring slots for every molecule
+            std::vector<double> c{1.0};
+            std::vector<int> sl{slot};
             if (mm >= opts.diis_start) {
-                std::vector<double> Bm((size_t)mm * mm), c;
+                std::vector<double> Bm((size_t)mm * mm), cd;
                 for (int i = 0; i < mm; ++i)
                     for (int j = 0; j < mm; ++j) Bm[(size_t)i * mm + j] = b.dotm[i][j];
-                if (diis_solve(mm, Bm, c)) {
-                    std::vector<int> sl(b.ring.begin(), b.ring.end());
-                    HIPCHK(hipMemcpyAsync(b.coef.p, c.data(), mm * sizeof(double), hipMemcpyHostToDevice, st));
-                    HIPCHK(hipMemcpyAsync(b.slots.p, sl.data(), mm * sizeof(int), hipMemcpyHostToDevice, st));
-                    HIPCHK(lincomb(nn, b.focks.as<double>(), nn, b.slots.as<int>(), b.coef.as<double>(), mm,
-                                   b.Fx.as<double>(), st));
-                    HIPCHK(hipStreamSynchronize(st));  // c, sl are host temporaries
-                    Fuse = b.Fx.as<double>();
+                if (diis_solve(mm, Bm, cd)) {
+                    c = cd;
+                    sl.assign(b.ring.begin(), b.ring.end());
                 }
             }
+            comb_at.push_back(coef_all.size());
+            coef_all.insert(coef_all.end(), c.begin(), c.end());
+            slot_all.insert(slot_all.end(), sl.begin(), sl.end());
+            comb.push_back(CombRec{nn, b.focks.as<double>(), nullptr, nullptr, (int)c.size(), b.Fx.as<double>()});
+            comb_len = std::max(comb_len, nn);
             if (opts.level_shift > 0.0 && b.err > opts.level_shift_until) {
-                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.S.as<double>(), n, b.P.as<double>(), n, 0.0, b.T1.as<double>(), n, st));
-                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, b.T1.as<double>(), n, b.S.as<double>(), n, 0.0, b.T2.as<double>(), n, st));
-                HIPCHK(level_shift_fock(nn, Fuse, b.S.as<double>(), b.T2.as<double>(), opts.level_shift, b.Fx.as<double>(), st));
-                Fuse = b.Fx.as<double>();
+                ls1.push_back(GemmRec{n, n, n, n, n, n, b.S.as<double>(), b.P.as<double>(), b.T1.as<double>()});
+                ls2.push_back(GemmRec{n, n, n, n, n, n, b.T1.as<double>(), b.S.as<double>(), b.T2.as<double>()});
+                shift.push_back(ShiftRec{nn, b.Fx.as<double>(), b.S.as<double>(), b.T2.as<double>()});
             }
-            if (Fuse != b.Fx.as<double>()) HIPCHK(copy_f64(nn, Fuse, b.Fx.as<double>(), st));
+        }
+        {
+            const void *d = nullptr;
+            if (!comb.empty()) {
+                const void *dc = nullptr, *ds = nullptr;
+                STCHK(stage(coef_all.data(), coef_all.size() * sizeof(double), &dc));
+                STCHK(stage(slot_all.data(), slot_all.size() * sizeof(int), &ds));
+                for (size_t i = 0; i < comb.size(); ++i) {
+                    comb[i].c = static_cast<const double *>(dc) + comb_at[i];
+                    comb[i].slots = static_cast<const int *>(ds) + comb_at[i];
+                }
+                STCHK(stage(comb.data(), comb.size() * sizeof(CombRec), &d));
+                HIPCHK(lincomb_batch(static_cast<const CombRec *>(d), (int)comb.size(), comb_len, st));
+            }
+            // level shift: virtual orbitals raised by sigma, F + sigma (S - S P S / 2)
+            STCHK(gemm_all(false, false, 1.0, ls1));
+            STCHK(gemm_all(false, false, 1.0, ls2));
+            if (!shift.empty()) {
+                STCHK(stage(shift.data(), shift.size() * sizeof(ShiftRec), &d));
+                HIPCHK(level_shift_batch(static_cast<const ShiftRec *>(d), (int)shift.size(), comb_len,
+                                         opts.level_shift, st));
+            }
         }
         phase_mark("diis");
         STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.Fx.as<double>(); }));
         phase_mark("diagonalize");
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (scf/src/lib.rs:223)
-        for (int m = 0; m < count; ++m) {
-            BatchMol &b = *mols[m];
-            if (b.done) continue;
-            HIPCHK(energy_trace(b.n, b.P.as<double>(), b.H.as<double>(), b.G.as<double>(), b.dots.as<double>() + 16, st));
-            HIPCHK(hipMemcpyAsync(hostd.data() + (size_t)m * 18 + 17, b.dots.as<double>() + 16, sizeof(double),
-                                  hipMemcpyDeviceToHost, st));
+        {
+            std::vector<TraceRec> tr;
+            for (int m = 0; m < count; ++m) {
+                BatchMol &b = *mols[m];
+                if (b.done) continue;
+                tr.push_back(TraceRec{b.n, b.P.as<double>(), b.H.as<double>(), b.G.as<double>(),
+                                      dots_all + (size_t)m * 18 + 17});
+            }
+            const void *d = nullptr;
+            STCHK(stage(tr.data(), tr.size() * sizeof(TraceRec), &d));
+            HIPCHK(energy_trace_batch(static_cast<const TraceRec *>(d), (int)tr.size(), st));
+            HIPCHK(hipMemcpyAsync(hostd.data(), dots_all, (size_t)count * 18 * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+            HIPCHK(hipStreamSynchronize(st));
         }
-        HIPCHK(hipStreamSynchronize(st));
         bool changed = false;
         for (int m = 0; m < count; ++m) {
             BatchMol &b = *mols[m];

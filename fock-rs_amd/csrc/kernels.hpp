@@ -131,6 +131,46 @@ hipError_t scatter_jk_batch(int nrec, const ScatterRec *recs, int max_npair, hip
 // J[f0(a) + c][f0(b) + d] += Jk[(c * nf(b) + d) * npair + j] for every pair j = (a, b), then Jk = 0
 hipError_t scatter_jk(int npair, const int2 *pair_shell, const int *shell_f0, const int *shell_nf, double *Jk,
                       double *J, int n, hipStream_t s);
+// Batched dense steps of the batched SCF: one launch per step for all molecules, one record each.
+struct GemmRec {  // C = alpha op(A) op(B) + beta C, op(A) m x k, op(B) k x n, row-major with leading dims
+    int m, n, k, lda, ldb, ldc;
+    const double *A, *B;
+    double *C;
+};
+hipError_t gemm_f64_batch(bool transA, bool transB, double alpha, double beta, const GemmRec *recs, int count,
+                          int max_m, int max_n, hipStream_t s);
+struct DiisRec {  // one molecule's DIIS error step
+    int n;
+    const double *FPS;   // F P S
+    double *e;           // out: FPS - (FPS)^T (the ring slot)
+    const double *F;
+    double *f_slot;      // out: copy of F (the ring slot)
+    const double *errs;  // the 16 error slots, n * n apart
+    double *dots;        // out: dots[s] = <e, errs[s]> (16), dots[16] = max |e| (zeroed beforehand)
+};
+hipError_t diis_error_batch(const DiisRec *recs, int count, int max_n, hipStream_t s);
+struct CombRec {  // out = sum_i c[i] base[slots[i]] (matrices of len doubles)
+    int64_t len;
+    const double *base;
+    const int *slots;
+    const double *c;
+    int count;
+    double *out;
+};
+hipError_t lincomb_batch(const CombRec *recs, int count, int64_t max_len, hipStream_t s);
+struct ShiftRec {  // F += sigma (S - SPS / 2)
+    int64_t len;
+    double *F;
+    const double *S, *SPS;
+};
+hipError_t level_shift_batch(const ShiftRec *recs, int count, int64_t max_len, double sigma, hipStream_t s);
+struct TraceRec {  // out = 1/2 sum P (2H + G)
+    int n;
+    const double *P, *H, *G;
+    double *out;
+};
+hipError_t energy_trace_batch(const TraceRec *recs, int count, hipStream_t s);
+
 // one matrix of a batched symmetric eigenproblem (n = 0: skipped)
 struct EigRec {
     int n;
