@@ -52,6 +52,17 @@ def test_batch_64_clusters(ctx):
     check_against_single(ctx, mols, res, o, [0, big, small])
 
 
+def test_batch_above_jacobi_limit_uses_rocsolver(ctx):
+    """n-decane/6-31G (n = 134 > 128, the LDS limit of the batched Jacobi solver) with two water
+    clusters: the batch falls back to rocSOLVER per size group and still matches single runs."""
+    mols = clusters(2, seed=7) + [m.flatten(m.build(m.alkane_geometry(10), "6-31G"))]
+    assert mols[-1][0].n == 134
+    o = opts()
+    st, statuses, res = ctx.scf_batch_run(mols, o)
+    assert st == abi.FOCK_OK, statuses
+    check_against_single(ctx, mols, res, o, range(len(mols)))
+
+
 def test_batch_rejects_mixed_precision(ctx):
     mols = clusters(2, seed=3)
     with pytest.raises(abi.FockError):
