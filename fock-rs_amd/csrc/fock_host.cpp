@@ -18,8 +18,9 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <atomic>
 #include <memory>
-#include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -136,7 +137,9 @@ struct fock_ctx {
     int batch_schedule = FOCK_SCHEDULE_COST;  // block order of batched builds
     bool timing = false;
     hipEvent_t tev[NUM_CLASSES + 1] = {};
+    std::mutex err_mu;  // preparation threads of a batch may fail at once
     fock_status fail(fock_status st, const std::string &msg) {
+        std::lock_guard<std::mutex> lock(err_mu);
         err = msg;
         return st;
     }
@@ -453,8 +456,13 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
 namespace {
 // f(lo, hi) over [0, n) in up to 8 host threads (pure host work: no HIP calls inside f); small
 // ranges run on the calling thread.
+thread_local bool tl_serial = false;  // set in worker threads: nested parallel loops run serially
 template <typename F>
 void parallel_for(int n, int grain, F f) {
+    if (tl_serial) {
+        f(0, n);
+        return;
+    }
     const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
     const int T = std::min({8, hw, n / std::max(grain, 1)});
     if (T <= 1) {
@@ -1666,9 +1674,35 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
                 const double d[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
                 bm->e_nuc += nu->charges[i] * nu->charges[j] / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             }
-        STCHK(prepare_into(ctx, bm->pr, bm->basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, 0));
         nmax = std::max(nmax, bm->n);
         mols.push_back(std::move(bm));
+    }
+    {
+        // per-molecule preparation in up to 8 threads (each writes only its molecule's state; the
+        // host work overlaps, the device work shares the context's stream)
+        std::vector<fock_status> pst(count, FOCK_OK);
+        std::atomic<int> next{0};
+        auto worker = [&]() {
+            tl_serial = true;
+            if (hipSetDevice(ctx->device) != hipSuccess) {
+                for (int m; (m = next++) < count;) pst[m] = FOCK_HIP_ERROR;
+                return;
+            }
+            for (int m; (m = next++) < count;)
+                pst[m] = prepare_into(ctx, mols[m]->pr, mols[m]->basis, opts.integral_policy, opts.screen,
+                                      opts.schwarz_threshold, 0);
+        };
+        const int T = std::min({8, count, (int)std::max(1u, std::thread::hardware_concurrency())});
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; ++t) th.emplace_back(worker);
+        worker();
+        tl_serial = false;
+        for (auto &x : th) x.join();
+        for (int m = 0; m < count; ++m)
+            if (pst[m] != FOCK_OK) {
+                if (pst[m] == FOCK_HIP_ERROR && ctx->err.empty()) ctx->err = "hipSetDevice failed in a worker";
+                return pst[m];
+            }
     }
     phase_mark("prepare");
     if (!blocked_fits(ctx, nmax, false, opts.integral_policy, ctx->tiles[FOCK_TILES_BATCH]))
