@@ -1677,20 +1677,15 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         nmax = std::max(nmax, bm->n);
         mols.push_back(std::move(bm));
     }
-    {
-        // per-molecule preparation in up to 8 threads (each writes only its molecule's state; the
-        // host work overlaps, the device work shares the context's stream)
+    // per-molecule work in up to 8 threads (each writes only its molecule's state; the host work
+    // and the allocations overlap, the device work shares the context's stream)
+    auto for_molecules = [&](auto fn) -> fock_status {
         std::vector<fock_status> pst(count, FOCK_OK);
         std::atomic<int> next{0};
         auto worker = [&]() {
             tl_serial = true;
-            if (hipSetDevice(ctx->device) != hipSuccess) {
-                for (int m; (m = next++) < count;) pst[m] = FOCK_HIP_ERROR;
-                return;
-            }
-            for (int m; (m = next++) < count;)
-                pst[m] = prepare_into(ctx, mols[m]->pr, mols[m]->basis, opts.integral_policy, opts.screen,
-                                      opts.schwarz_threshold, 0);
+            const bool dev_ok = hipSetDevice(ctx->device) == hipSuccess;
+            for (int m; (m = next++) < count;) pst[m] = dev_ok ? fn(*mols[m]) : FOCK_HIP_ERROR;
         };
         const int T = std::min({8, count, (int)std::max(1u, std::thread::hardware_concurrency())});
         std::vector<std::thread> th;
@@ -1703,7 +1698,11 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
                 if (pst[m] == FOCK_HIP_ERROR && ctx->err.empty()) ctx->err = "hipSetDevice failed in a worker";
                 return pst[m];
             }
-    }
+        return FOCK_OK;
+    };
+    STCHK(for_molecules([&](BatchMol &b) -> fock_status {
+        return prepare_into(ctx, b.pr, b.basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, 0);
+    }));
     phase_mark("prepare");
     if (!blocked_fits(ctx, nmax, false, opts.integral_policy, ctx->tiles[FOCK_TILES_BATCH]))
         return ctx->fail(FOCK_UNSUPPORTED, "largest basis too large for a batch");
@@ -1735,8 +1734,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     auto gA = [&](BatchMol &b) { return gr[b.group]->A.as<double>() + (size_t)b.gidx * b.n * b.n; };
     auto gW = [&](BatchMol &b) { return gr[b.group]->W.as<double>() + (size_t)b.gidx * b.n; };
     // per-molecule matrices, S/T/V, H
-    for (auto &bp : mols) {
-        BatchMol &b = *bp;
+    STCHK(for_molecules([&](BatchMol &b) -> fock_status {
         const size_t nn = (size_t)b.n * b.n, bytes = nn * sizeof(double);
         for (DevBuf *d : {&b.S, &b.T, &b.V, &b.H, &b.X, &b.U, &b.T1, &b.T2, &b.P, &b.G, &b.F, &b.Fx, &b.C})
             HIPCHK(d->ensure(bytes));
@@ -1748,9 +1746,10 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         STCHK(enqueue_one_electron(ctx, b.basis, b.nuc, bd, opts.integral_policy, b.S.as<double>(), b.T.as<double>(),
                                    b.V.as<double>()));
         HIPCHK(add_f64((int64_t)nn, b.T.as<double>(), b.V.as<double>(), b.H.as<double>(), st));
-        HIPCHK(hipStreamSynchronize(st));  // bd goes out of scope
         HIPCHK(copy_f64((int64_t)nn, b.S.as<double>(), gA(b), st));
-    }
+        HIPCHK(hipStreamSynchronize(st));  // bd goes out of scope
+        return FOCK_OK;
+    }));
     // Small per-launch tables (records of the batched steps, DIIS coefficients) go through a pinned
     // host buffer mirrored on the device, bump-allocated and reset only after a synchronisation of
     // the stream (when every copy out of it has completed).
