@@ -1606,6 +1606,8 @@ struct BatchMol {
     int n = 0, nocc = 0, group = 0, gidx = 0;
     double e_nuc = 0.0;
     DevBuf S, T, V, H, X, U, T1, T2, P, G, F, Fx, C, errs, focks;
+    DevBuf E;             // eigenvectors of the last F' (row k: the k-th), the next solve's start
+    bool have_e = false;
     std::deque<int> ring;                 // stored (error, Fock) slots, oldest first
     std::deque<std::vector<double>> dotm;  // <e_i, e_j> of the stored slots
     int next_slot = 0;
@@ -1736,7 +1738,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     // per-molecule matrices, S/T/V, H
     STCHK(for_molecules([&](BatchMol &b) -> fock_status {
         const size_t nn = (size_t)b.n * b.n, bytes = nn * sizeof(double);
-        for (DevBuf *d : {&b.S, &b.T, &b.V, &b.H, &b.X, &b.U, &b.T1, &b.T2, &b.P, &b.G, &b.F, &b.Fx, &b.C})
+        for (DevBuf *d : {&b.S, &b.T, &b.V, &b.H, &b.X, &b.U, &b.T1, &b.T2, &b.P, &b.G, &b.F, &b.Fx, &b.C, &b.E})
             HIPCHK(d->ensure(bytes));
         HIPCHK(b.errs.ensure(16 * bytes));
         HIPCHK(b.focks.ensure(16 * bytes));
@@ -1823,22 +1825,43 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U, n, 0.0, b.X.as<double>(), n, st));
     }
     // F' = X^T M X for every molecule into its group slot, batched eigensolve, C = X R'^T,
-    // P = 2 C_occ C_occ^T: one launch per step for all unconverged molecules
+    // P = 2 C_occ C_occ^T: one launch per step for all unconverged molecules. From the second solve
+    // on, the eigenproblem is taken in the basis of the previous eigenvectors E (M = E F' E^T is
+    // nearly diagonal, so the Jacobi sweeps converge in a few) and its vectors turned back with E.
     auto diagonalize_all = [&](auto fock_of) -> fock_status {
-        std::vector<GemmRec> g1, g2, g3, g4;
+        std::vector<GemmRec> g1, g2, w1, w2, w3, g3, g4;
         for (auto &bp : mols) {
             BatchMol &b = *bp;
             if (b.done) continue;
             const int n = b.n;
             g1.push_back(GemmRec{n, n, n, n, n, n, fock_of(b), b.X.as<double>(), b.T1.as<double>()});
             g2.push_back(GemmRec{n, n, n, n, n, n, b.X.as<double>(), b.T1.as<double>(), gA(b)});
-            g3.push_back(GemmRec{n, n, n, n, n, n, b.X.as<double>(), gA(b), b.C.as<double>()});
-            g4.push_back(GemmRec{n, n, b.nocc, n, n, n, b.C.as<double>(), b.C.as<double>(), b.P.as<double>()});
+            if (b.have_e) {
+                w1.push_back(GemmRec{n, n, n, n, n, n, gA(b), b.E.as<double>(), b.T2.as<double>()});   // F' E^T
+                w2.push_back(GemmRec{n, n, n, n, n, n, b.E.as<double>(), b.T2.as<double>(), gA(b)});   // E (F' E^T)
+                w3.push_back(GemmRec{n, n, n, n, n, n, gA(b), b.E.as<double>(), b.U.as<double>()});    // E_M E
+            }
         }
         STCHK(gemm_all(false, false, 1.0, g1));  // F X
         STCHK(gemm_all(true, false, 1.0, g2));   // X^T (F X)
+        STCHK(gemm_all(false, true, 1.0, w1));
+        STCHK(gemm_all(false, false, 1.0, w2));
         STCHK(eig_all(true));
-        STCHK(gemm_all(false, true, 1.0, g3));   // C = X R'^T
+        STCHK(gemm_all(false, false, 1.0, w3));
+        for (auto &bp : mols) {
+            BatchMol &b = *bp;
+            if (b.done) continue;
+            const int n = b.n;
+            if (b.have_e) {  // the turned-back vectors become E (U is free after X was formed)
+                std::swap(b.E.p, b.U.p);
+                std::swap(b.E.bytes, b.U.bytes);
+            }
+            else HIPCHK(copy_f64((int64_t)n * n, gA(b), b.E.as<double>(), st));
+            b.have_e = true;
+            g3.push_back(GemmRec{n, n, n, n, n, n, b.X.as<double>(), b.E.as<double>(), b.C.as<double>()});
+            g4.push_back(GemmRec{n, n, b.nocc, n, n, n, b.C.as<double>(), b.C.as<double>(), b.P.as<double>()});
+        }
+        STCHK(gemm_all(false, true, 1.0, g3));   // C = X E^T
         STCHK(gemm_all(false, true, 2.0, g4));   // P = 2 C_occ C_occ^T (scf/src/lib.rs:214-216)
         return FOCK_OK;
     };
