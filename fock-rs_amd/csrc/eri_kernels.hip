@@ -493,10 +493,8 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     double *red = Kb + NB * n;        // [4][NAB]
     const int tile = args_in.tile;  // most tasks per block of this launch (blk.ntasks <= tile)
     Task *s_task = reinterpret_cast<Task *>(red + 4 * NAB);         // [tile]
-    PairInfo *s_ket = reinterpret_cast<PairInfo *>(s_task + tile);  // [tile * 64] kept kets (SoA starts)
-    int2 *s_kx = reinterpret_cast<int2 *>(s_ket + tile * 64);       // [tile * 64] (pair index, bucket size)
-    int *s_stride = reinterpret_cast<int *>(s_kx + tile * 64);      // [tile] bucket sizes (2 tile ints: alignment)
-    unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_stride + 2 * tile);  // [tile]
+    int *s_kidx = reinterpret_cast<int *>(s_task + tile);  // [tile * 64] kept ket pair indices
+    unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_kidx + tile * 64);  // [tile]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
@@ -504,7 +502,6 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     if (threadIdx.x < blk.ntasks) {
         const Task tk = args.tasks[blk.first_task + threadIdx.x];
         s_task[threadIdx.x] = tk;
-        s_stride[threadIdx.x] = args.soa_stride[tk.ket_begin];
     }
     __syncthreads();
     // the block's kept kets, packed densely in task order: wave w owns task chunks w, w + 4 (a
@@ -541,9 +538,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         if ((m >> lane) & 1ull) {
             int r = __popcll(m & ((1ull << lane) - 1ull));
             for (int u = 0; u < t; ++u) r += __popcll(s_mask[u]);
-            const Task tk = s_task[t];
-            s_ket[r] = args.soa_pairs[tk.ket_begin + lane];
-            s_kx[r] = make_int2(tk.ket_begin + lane, s_stride[t]);
+            s_kidx[r] = s_task[t].ket_begin + lane;
         }
     }
     for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
@@ -577,11 +572,13 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     for (int g = wave * 64; g < kept; g += 256) {
         const bool active = g + lane < kept;
         const int slot = active ? g + lane : g;
-        const PairInfo ket = s_ket[slot];
-        const int2 kx = s_kx[slot];
-        const int ket_idx = kx.x;
+        // the ket pair's SoA start and bucket size from global memory (consecutive lanes hold
+        // mostly consecutive pairs: coalesced, L2-resident), so LDS keeps 4 bytes per kept ket
+        const int ket_idx = s_kidx[slot];
+        const PairInfo ket = args.soa_pairs[ket_idx];
+        const int ket_stride = args.soa_stride[ket_idx];
         Real I[NAB][NCD];
-        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, kx.y, active,
+        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, ket_stride, active,
                                                                     POLICY == 1 ? slice : nullptr, I);
         const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
@@ -714,8 +711,7 @@ size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile) {
     const size_t cols = (fp32 ? BOYS_TAYLOR32 : BOYS_TAYLOR64) + 1, real = fp32 ? sizeof(float) : sizeof(double);
     const size_t pp = fp32 ? sizeof(PrimPair<float>) : sizeof(PrimPair<double>);
     return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
-           (size_t)tile * (sizeof(Task) + 64 * (sizeof(PairInfo) + sizeof(int2)) + 2 * sizeof(int) +
-                           sizeof(unsigned long long)) +
+           (size_t)tile * (sizeof(Task) + 64 * sizeof(int) + sizeof(unsigned long long)) +
            MAX_PRIM_PAIRS * (pp + sizeof(float)) +
            (policy == 1 ? BOYS_GRID_N * cols * real : 0);
 }

@@ -93,7 +93,7 @@ constexpr int MAX_TILE = 16;
 constexpr int NUM_TILE_CHOICES = 5;
 constexpr int TILE_CHOICES[NUM_TILE_CHOICES] = {1, 2, 4, 8, 16};
 // largest basis for the blocked digestion: (3 + 3) K rows of n doubles in LDS next to the staged
-// Boys slab (at most 33 KB) and the block's ket records (1.6 KB per task of the tile)
+// Boys slab (at most 33 KB) and the block's ket records (0.3 KB per task of the tile)
 constexpr int BLOCKED_MAX_NBF = 600;
 
 // density-weighted screening of the static task list (compact.hip)
