@@ -450,7 +450,8 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
     constexpr int L = LA + LB + LC + LD;
-    constexpr int COLS = (sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32) + 1;
+    constexpr int COLS = sizeof(Real) == 8 ? BOYS_PD_COLS64 : BOYS_PD_COLS32;
+    constexpr int GN = sizeof(Real) == 8 ? BOYS_GRID_N : BOYS_PD32_N;  // rows of the Boys slab
     static_assert(MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) % 16 == 0, "LDS slab alignment");
     // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | J_ab partials of the 4 waves |
     // the block's tasks | its kept ket pairs
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     PrimPair<Real> *bra_pp = reinterpret_cast<PrimPair<Real> *>(lds_bytes);  // [MAX_PRIM_PAIRS]
     float *bra_q = reinterpret_cast<float *>(bra_pp + MAX_PRIM_PAIRS);       // [MAX_PRIM_PAIRS]
     Real *slice = reinterpret_cast<Real *>(bra_q + MAX_PRIM_PAIRS);          // [BOYS_GRID_N][COLS]
-    double *lds = reinterpret_cast<double *>(slice + (POLICY == 1 ? BOYS_GRID_N * COLS : 0));
+    double *lds = reinterpret_cast<double *>(slice + (POLICY == 1 ? (GN * COLS + 1) / 2 * 2 : 0));
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
     double *red = Kb + NB * n;        // [4][NAB]
@@ -552,8 +553,8 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
             const Real *slab;
             if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
             else slab = reinterpret_cast<const Real *>(args.bt.pd32);
-            slab += (size_t)L * BOYS_GRID_N * COLS;
-            for (int e = threadIdx.x; e < BOYS_GRID_N * COLS; e += blockDim.x) slice[e] = slab[e];
+            slab += (size_t)L * GN * COLS;
+            for (int e = threadIdx.x; e < GN * COLS; e += blockDim.x) slice[e] = slab[e];
         }
     }
     __syncthreads();
@@ -708,12 +709,13 @@ static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool ref
 // task and kept-ket records, the bra primitive pairs with their q, and the class's Boys slab
 size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile) {
     const int na = CLASS_BRA[cls] == 0 ? 1 : 3, nb = CLASS_BRA[cls] == 2 ? 3 : 1;
-    const size_t cols = (fp32 ? BOYS_TAYLOR32 : BOYS_TAYLOR64) + 1, real = fp32 ? sizeof(float) : sizeof(double);
+    const size_t cols = fp32 ? BOYS_PD_COLS32 : BOYS_PD_COLS64, real = fp32 ? sizeof(float) : sizeof(double);
+    const size_t rows = fp32 ? BOYS_PD32_N : BOYS_GRID_N;
     const size_t pp = fp32 ? sizeof(PrimPair<float>) : sizeof(PrimPair<double>);
     return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
            (size_t)tile * (sizeof(Task) + 64 * sizeof(int) + sizeof(unsigned long long)) +
            MAX_PRIM_PAIRS * (pp + sizeof(float)) +
-           (policy == 1 ? BOYS_GRID_N * cols * real : 0);
+           (policy == 1 ? (rows * cols + 1) / 2 * 2 * real : 0);
 }
 
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool BATCH = false>

@@ -92,14 +92,20 @@ constexpr int BOYS_TAYLOR32 = 5;        // terms (k = 0..4): < 0.05^5/5! ~ 3e-9 
 // expansion is a plain Horner chain of FMAs. One [BOYS_GRID_N][PD_COLS] slab per L <= PD_LMAX.
 constexpr int BOYS_PD_LMAX = 4;
 constexpr int BOYS_PD_COLS64 = BOYS_TAYLOR64 + 1;
-constexpr int BOYS_PD_COLS32 = BOYS_TAYLOR32 + 1;
+// The FP32 slabs (staged in LDS by the FP32 block kernels) use a grid twice as coarse with one more
+// term: T_i = 0.2 i, i = 0..225, |d| <= 0.1, error < 0.1^6/6! ~ 1.4e-9 relative, and 6.3 KB per
+// slab instead of 10.8 KB (LDS is what bounds the FP32 kernels' occupancy)
+constexpr int BOYS_PD32_N = 226;
+constexpr double BOYS_PD32_STEP = 0.2;
+constexpr int BOYS_PD32_TERMS = 6;
+constexpr int BOYS_PD_COLS32 = BOYS_PD32_TERMS + 1;
 
 struct BoysTables {
     const double *lut_ref;  // the reference's 200-entry F_10 table
     const double *tab64;    // [BOYS_GRID_N][BOYS_ORDERS]
     const float *tab32;     // [BOYS_GRID_N][BOYS_ORDERS]
     const double *pd64;     // [BOYS_PD_LMAX + 1][BOYS_GRID_N][BOYS_PD_COLS64]
-    const float *pd32;      // [BOYS_PD_LMAX + 1][BOYS_GRID_N][BOYS_PD_COLS32]
+    const float *pd32;      // [BOYS_PD_LMAX + 1][BOYS_PD32_N][BOYS_PD_COLS32]
 };
 
 __device__ __forceinline__ double rcp_exact(double x) { return 1.0 / x; }
@@ -244,13 +250,13 @@ __device__ __forceinline__ void boys_taylor32(float T, const float *__restrict__
 // F_L, exp(-T) = exp(-T_i) e^{T_i - T} with the 1/k! constants, then downward recursion.
 template <int L, typename Real>
 __device__ __forceinline__ void boys_taylor_lds(Real T, const Real *__restrict__ slab, Real F[L + 1]) {
-    constexpr int TERMS = sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_TAYLOR32;
+    constexpr int TERMS = sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_PD32_TERMS;
     constexpr int COLS = TERMS + 1;
     if (T < Real(BOYS_TMAX)) {
         int i;
         if constexpr (sizeof(Real) == 8) i = (int)fmax(T * 10.0 + 0.5, 0.0);
-        else i = (int)fmaxf(T * 10.0f + 0.5f, 0.0f);
-        const Real d = (Real)i * Real(BOYS_GRID_STEP) - T;
+        else i = (int)fmaxf(T * 5.0f + 0.5f, 0.0f);
+        const Real d = (Real)i * Real(sizeof(Real) == 8 ? BOYS_GRID_STEP : BOYS_PD32_STEP) - T;
         const Real *row = slab + i * COLS;
         Real acc = row[TERMS - 1];
 #pragma unroll
@@ -263,8 +269,9 @@ __device__ __forceinline__ void boys_taylor_lds(Real T, const Real *__restrict__
                 ed = ed * d + Real(1.0 / 720.0);
                 ed = ed * d + Real(1.0 / 120.0);
                 ed = ed * d + Real(1.0 / 24.0);
-            } else {
-                ed = Real(1.0 / 24.0);
+            } else {  // |d| <= 0.1: through d^5 / 5!
+                ed = Real(1.0 / 120.0);
+                ed = ed * d + Real(1.0 / 24.0);
             }
             ed = ed * d + Real(1.0 / 6.0);
             ed = ed * d + Real(0.5);

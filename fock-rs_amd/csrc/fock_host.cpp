@@ -172,20 +172,21 @@ long double boys_series(long double T, int m) {
     return expl(-T) * sum;
 }
 
-// Pre-divided slabs: [L][i][k] = F_{L+k}(T_i) / k! (k < terms), [L][i][terms] = exp(-T_i).
+// Pre-divided slabs on the grid T_i = i / per_unit (i < rows): [L][i][k] = F_{L+k}(T_i) / k!
+// (k < terms), [L][i][terms] = exp(-T_i).
 template <typename Real>
-void make_boys_pd(std::vector<Real> &out, int terms) {
+void make_boys_pd(std::vector<Real> &out, int terms, int rows, int per_unit) {
     const int cols = terms + 1;
-    out.assign((size_t)(BOYS_PD_LMAX + 1) * BOYS_GRID_N * cols, Real(0));
+    out.assign((size_t)(BOYS_PD_LMAX + 1) * rows * cols, Real(0));
     for (int L = 0; L <= BOYS_PD_LMAX; ++L)
-        for (int i = 0; i < BOYS_GRID_N; ++i) {
-            const long double T = (long double)i / 10.0L;
+        for (int i = 0; i < rows; ++i) {
+            const long double T = (long double)i / (long double)per_unit;
             long double fact = 1.0L;
             for (int k = 0; k < terms; ++k) {
                 if (k > 0) fact *= k;
-                out[((size_t)L * BOYS_GRID_N + i) * cols + k] = (Real)(boys_series(T, L + k) / fact);
+                out[((size_t)L * rows + i) * cols + k] = (Real)(boys_series(T, L + k) / fact);
             }
-            out[((size_t)L * BOYS_GRID_N + i) * cols + terms] = (Real)expl(-T);
+            out[((size_t)L * rows + i) * cols + terms] = (Real)expl(-T);
         }
 }
 
@@ -352,8 +353,8 @@ extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
     HIPCHK(hipMemcpy(ctx->tab32.p, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
     std::vector<double> p64;
     std::vector<float> p32;
-    make_boys_pd(p64, BOYS_TAYLOR64);
-    make_boys_pd(p32, BOYS_TAYLOR32);
+    make_boys_pd(p64, BOYS_TAYLOR64, BOYS_GRID_N, 10);
+    make_boys_pd(p32, BOYS_PD32_TERMS, BOYS_PD32_N, 5);
     HIPCHK(ctx->pd64.ensure(p64.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->pd64.p, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(ctx->pd32.ensure(p32.size() * sizeof(float)));
