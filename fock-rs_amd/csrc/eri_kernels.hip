@@ -452,6 +452,9 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     constexpr int L = LA + LB + LC + LD;
     constexpr int COLS = sizeof(Real) == 8 ? BOYS_PD_COLS64 : BOYS_PD_COLS32;
     constexpr int GN = sizeof(Real) == 8 ? BOYS_GRID_N : BOYS_PD32_N;  // rows of the Boys slab
+    // the slab is staged in LDS for the FP32 kernels and FP64 (ss|ss); the other FP64 classes read
+    // the Boys table from global memory (their 33 KB slab cost more occupancy than the L2 reads)
+    constexpr bool SLAB = POLICY == 1 && block_slab_in_lds(L, sizeof(Real) == 4);
     static_assert(MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) % 16 == 0, "LDS slab alignment");
     // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | J_ab partials of the 4 waves |
     // the block's tasks | its kept ket pairs
@@ -488,7 +491,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     PrimPair<Real> *bra_pp = reinterpret_cast<PrimPair<Real> *>(lds_bytes);  // [MAX_PRIM_PAIRS]
     float *bra_q = reinterpret_cast<float *>(bra_pp + MAX_PRIM_PAIRS);       // [MAX_PRIM_PAIRS]
     Real *slice = reinterpret_cast<Real *>(bra_q + MAX_PRIM_PAIRS);          // [BOYS_GRID_N][COLS]
-    double *lds = reinterpret_cast<double *>(slice + (POLICY == 1 ? (GN * COLS + 1) / 2 * 2 : 0));
+    double *lds = reinterpret_cast<double *>(slice + (SLAB ? (GN * COLS + 1) / 2 * 2 : 0));
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
     double *red = Kb + NB * n;        // [4][NAB]
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
             bra_pp[e] = gpp[e];
             bra_q[e] = args.prim_q ? args.prim_q[bra.pp_start + e] : 0.0f;
         }
-        if constexpr (POLICY == 1) {
+        if constexpr (SLAB) {
             const Real *slab;
             if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
             else slab = reinterpret_cast<const Real *>(args.bt.pd32);
@@ -580,7 +583,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         const int ket_stride = args.soa_stride[ket_idx];
         Real I[NAB][NCD];
         lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, ket_stride, active,
-                                                                    POLICY == 1 ? slice : nullptr, I);
+                                                                    SLAB ? slice : nullptr, I);
         const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
         if (fa == fb) w *= Real(0.5);
@@ -715,7 +718,7 @@ size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile) {
     return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
            (size_t)tile * (sizeof(Task) + 64 * sizeof(int) + sizeof(unsigned long long)) +
            MAX_PRIM_PAIRS * (pp + sizeof(float)) +
-           (policy == 1 ? (rows * cols + 1) / 2 * 2 * real : 0);
+           ((policy == 1 && block_slab_in_lds(CLASS_BRA[cls] + CLASS_KET[cls], fp32)) ? (rows * cols + 1) / 2 * 2 * real : 0);
 }
 
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool BATCH = false>

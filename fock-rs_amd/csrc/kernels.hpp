@@ -95,6 +95,8 @@ constexpr int TILE_CHOICES[NUM_TILE_CHOICES] = {1, 2, 4, 8, 16};
 // largest basis for the blocked digestion: (3 + 3) K rows of n doubles in LDS next to the staged
 // Boys slab (at most 33 KB) and the block's ket records (0.3 KB per task of the tile)
 constexpr int BLOCKED_MAX_NBF = 600;
+// whether the blocked kernel of total angular momentum L stages its Boys slab in LDS (accurate policy)
+constexpr bool block_slab_in_lds(int L, bool fp32) { return fp32 || L == 0; }
 
 // density-weighted screening of the static task list (compact.hip)
 struct ScreenArgs {
