@@ -2557,7 +2557,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     // FP64 rebuild discards), tight in phase 1; bounds >= INC_TAU64 run in FP64 there (the FP32
     // error of such a quartet, ~1e-7 of its bound, stays below 1e-11)
     constexpr float INC_TAU_BULK = 1e-9f, INC_TAU = 1e-11f, INC_TAU64 = 1e-4f;
-    constexpr double MIXED_SWITCH = 1e-4;
+    constexpr double MIXED_SWITCH = 1e-4, MIXED_STALL = 1e-2;
     int since_rebuild = 0, stall = 0;
     bool need_full = false;  // next build of phase 1 is an FP64 full build
     double best_err = HUGE_VAL;
@@ -2717,7 +2717,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             if (err < best_err * 0.5) {
                 best_err = err;
                 stall = 0;
-            } else {
+            } else if (err < MIXED_STALL) {
+                // stagnation counts only near convergence (the FP32 noise floor is ~1e-6): early
+                // iterations that shrink the error slowly stay in the cheap FP32 phase
                 ++stall;
             }
             const bool near = err < std::max(10.0 * opts.epsilon, MIXED_SWITCH);
