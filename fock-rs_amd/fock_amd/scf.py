@@ -93,19 +93,20 @@ def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_char
             schwarz_threshold: float = 1e-12, precision: int = abi.PRECISION_FP64,
             incremental: bool = False, ctx: Optional[abi.Context] = None,
             initial_density: Optional[np.ndarray] = None, diis_start: int = 5, diis_window: int = 16,
-            level_shift: float = 0.0, level_shift_until: float = 0.0) -> Optional[HartreeFockResult]:
+            level_shift: float = 0.0, level_shift_until: float = 0.0,
+            autotune: bool = False) -> Optional[HartreeFockResult]:
     """SelfConsistentField::try_scf (scf/src/lib.rs:73-258) on the GPU.
 
     Defaults reproduce the reference's own arithmetic and schedule (REF_COMPAT integrals, its
     1e-6 basis-function-pair screen, FP64, GWH guess, DIIS from 5 stored matrices in a window
-    of 16)."""
+    of 16). autotune: time the J/K tiles at the guess density first (kept in the context)."""
     atoms = list(atoms)
     fb, fn = flatten(atoms)
     opts = abi.default_opts(max_iters=max_iters, epsilon=epsilon, molecule_charge=molecule_charge,
                             integral_policy=policy, screen=screen, schwarz_threshold=schwarz_threshold,
                             precision=precision, incremental=1 if incremental else 0,
                             diis_start=diis_start, diis_window=diis_window, level_shift=level_shift,
-                            level_shift_until=level_shift_until)
+                            level_shift_until=level_shift_until, autotune=1 if autotune else 0)
     if initial_density is not None:
         P0 = np.ascontiguousarray(initial_density, dtype=np.float64)
         if P0.shape != (fb.n, fb.n):

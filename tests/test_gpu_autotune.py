@@ -77,6 +77,17 @@ def test_jk_autotune_report_and_result(ctx):
     ctx.set_tiles(abi.TILES_FP64, [8] * 6)
 
 
+def test_facade_try_scf_autotune(ctx):
+    from fock_amd.scf import try_scf
+    atoms = m.build(m.water_cluster_geometry(3, np.random.default_rng(2)), "6-31G")
+    kw = dict(policy=abi.POLICY_ACCURATE, screen=abi.SCREEN_SCHWARZ, ctx=ctx, **ROBUST)
+    ctx.set_tiles(abi.TILES_FP64, [8] * 6)
+    r0 = try_scf(atoms, 200, 1e-10, 0, **kw)
+    r1 = try_scf(atoms, 200, 1e-10, 0, autotune=True, **kw)
+    assert r0.converged and r1.converged and abs(r0.total_energy - r1.total_energy) < 1e-10
+    ctx.set_tiles(abi.TILES_FP64, [8] * 6)
+
+
 def test_scf_autotune_matches_untuned(ctx):
     fb, fn = cluster(4, 4)
     o = abi.default_opts(max_iters=200, epsilon=1e-10, integral_policy=abi.POLICY_ACCURATE,
