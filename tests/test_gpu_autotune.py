@@ -42,12 +42,16 @@ def test_tiles_default_and_validation(ctx):
         ctx.set_tiles(5, [8] * 6)
 
 
-@pytest.mark.parametrize("precision,tol", [(abi.PRECISION_FP64, 1e-12), (abi.PRECISION_MIXED, 1e-9)])
-def test_every_tile_gives_the_same_jk(ctx, precision, tol):
+@pytest.mark.parametrize("policy,screen,precision,tol", [
+    (abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, abi.PRECISION_FP64, 1e-12),
+    (abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, abi.PRECISION_MIXED, 1e-9),
+    (abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR, abi.PRECISION_FP64, 1e-12),
+])
+def test_every_tile_gives_the_same_jk(ctx, policy, screen, precision, tol):
     fb, _ = cluster(6, 7)
     form = abi.TILES_FP64 if precision == abi.PRECISION_FP64 else abi.TILES_MIXED
     ctx.set_tiles(form, [8] * 6)
-    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12, precision)
+    ctx.jk_prepare(fb, policy, screen, 1e-12, precision)
     P = sym_density(fb.n, 3)
     J0, K0 = ctx.jk_build(P)
     for t in abi.TILE_CHOICES:
