@@ -1762,7 +1762,9 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             if (h) (void)hipHostFree(h);
         }
     } stg;
-    constexpr size_t STG_CAP = (size_t)1 << 21;
+    // per molecule and iteration the tables take under 1 KB (records of ~10 launches, 16 DIIS
+    // coefficients and slots), twice that is reserved
+    const size_t STG_CAP = std::max<size_t>((size_t)1 << 21, (size_t)count * 2048);
     DevBuf d_stg;
     size_t stg_off = 0;
     HIPCHK(hipHostMalloc((void **)&stg.h, STG_CAP, hipHostMallocDefault));
