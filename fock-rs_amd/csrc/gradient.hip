@@ -287,8 +287,10 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
         if (sg[e] != 0.0) unsafeAtomicAdd(&args.grad[e], sg[e]);
 }
 
-// One-electron part: one thread per shell pair (a, b), all component pairs; S and T via
-// d/dB = -d/dA, the attraction to every nucleus C with d/dC = -(d/dA + d/dB).
+// One-electron part: one thread per (shell pair (a, b), term), all component pairs. Threads
+// [0, npairs * natoms) take the attraction to nucleus C = t % natoms of pair t / natoms (d/dC =
+// -(d/dA + d/dB)); the last npairs threads take overlap and kinetic of one pair each (d/dB = -d/dA).
+// Neighbouring lanes share a pair (same primitive loop bounds) and a wave takes one kind of term.
 template <int LA, int LB>
 __global__ __launch_bounds__(128) void grad_one_kernel(GradOneArgs args) {
     constexpr int NA = ncart(LA), NB = ncart(LB);
@@ -296,9 +298,12 @@ __global__ __launch_bounds__(128) void grad_one_kernel(GradOneArgs args) {
     extern __shared__ double sg[];  // [3 * natoms]
     for (int e = threadIdx.x; e < 3 * args.natoms; e += blockDim.x) sg[e] = 0.0;
     __syncthreads();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < args.npairs) {
-        const int sa = args.pair_a[t], sb = args.pair_b[t];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nv = (long)args.npairs * args.natoms;
+    if (t < nv + args.npairs) {
+        const int pr = t < nv ? (int)(t / args.natoms) : (int)(t - nv);
+        const int c = t < nv ? (int)(t % args.natoms) : -1;  // -1: overlap and kinetic
+        const int sa = args.pair_a[pr], sb = args.pair_b[pr];
         const int fa = args.shell_f0[sa], fb = args.shell_f0[sb];
         const int atA = args.shell_atom[sa], atB = args.shell_atom[sb];
         const double A[3] = {args.centers[3 * fa], args.centers[3 * fa + 1], args.centers[3 * fa + 2]};
@@ -317,8 +322,7 @@ __global__ __launch_bounds__(128) void grad_one_kernel(GradOneArgs args) {
         const int ka0 = args.prim_off[fa], ka1 = args.prim_off[fa + 1];
         const int kb0 = args.prim_off[fb], kb1 = args.prim_off[fb + 1];
         double gA[3] = {0, 0, 0}, gB[3] = {0, 0, 0};
-        // nucleus -1: overlap and kinetic; nucleus c >= 0: attraction to c
-        for (int c = -1; c < args.natoms; ++c) {
+        {
             double gCn[3] = {0, 0, 0};
             for (int ia = ka0; ia < ka1; ++ia) {
                 const double a = args.exps[ia];
@@ -483,7 +487,8 @@ hipError_t launch_grad_eri(int cls, const GradEriArgs &a, hipStream_t s) {
 
 hipError_t launch_grad_one(int pair_class, const GradOneArgs &a, hipStream_t s) {
     if (a.npairs <= 0) return hipSuccess;
-    const dim3 grid((a.npairs + 127) / 128), block(128);
+    const size_t threads = (size_t)a.npairs * ((size_t)a.natoms + 1);
+    const dim3 grid((unsigned)((threads + 127) / 128)), block(128);
     const size_t lds = (size_t)3 * a.natoms * sizeof(double);
     switch (pair_class) {
         case 0: hipLaunchKernelGGL((grad_one_kernel<0, 0>), grid, block, lds, s, a); break;
