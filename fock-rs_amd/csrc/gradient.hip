@@ -203,7 +203,9 @@ __device__ __forceinline__ void accumulate_grad_quartet(const PrimPair<Real> &br
     });
 }
 
-// Two-electron part: one lane per unique shell quartet of the Fock build's task list.
+// Two-electron part: one lane per unique shell quartet of the Fock build's task list; args.split
+// consecutive waves share one task, wave s of them taking the primitive quartets s, s + split, ...
+// (derivative integrals are linear in the primitive quartets).
 template <int LA, int LB, int LC, int LD, typename Real>
 __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -212,7 +214,8 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
     for (int e = threadIdx.x; e < 3 * args.natoms; e += blockDim.x) sg[e] = 0.0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int t = blockIdx.x * 4 + wave;
+    const int gw = blockIdx.x * 4 + wave, split = args.split;
+    const int t = gw / split, part = gw - t * split;
     bool run = t < args.ntasks;
     Task task{};
     if (run) {
@@ -257,14 +260,38 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
                         }
             const PrimPair<Real> *__restrict__ prims = reinterpret_cast<const PrimPair<Real> *>(
                 sizeof(Real) == 8 ? (const void *)args.prims : (const void *)args.prims32);
-            for (int i = 0; i < bra.K; ++i) {
-                const PrimPair<Real> bp = prims[bra.pp_start + i];
-                const double2 ab = args.prim_ab[bra.pp_start + i];
-                for (int k = 0; k < ket.K; ++k) {
+            // primitive quartets ik = i * ket.K + k = part, part + split, ... (split = 1: all). The
+            // classes of total angular momentum >= 3 compile to fewer registers as one flat loop
+            // (measured on C8H18/6-31G: (pp|ps) 0.90 against 2.1 ms), the others as a walk that
+            // loads each bra primitive once
+            if constexpr (LA + LB + LC + LD >= 3) {
+                const int nq = bra.K * ket.K;
+                for (int ik = part; ik < nq; ik += split) {
+                    const int i = ik / ket.K, k = ik - i * ket.K;
+                    const PrimPair<Real> bp = prims[bra.pp_start + i];
+                    const double2 ab = args.prim_ab[bra.pp_start + i];
                     const PrimPair<Real> kp = prims[ket.pp_start + k];
                     const double2 cd = args.prim_ab[ket.pp_start + k];
-                    accumulate_grad_quartet<LA, LB, LC, LD, Real>(bp, kp, (Real)ab.x, (Real)ab.y, (Real)cd.x, args.bt,
-                                                                  fac, g);
+                    accumulate_grad_quartet<LA, LB, LC, LD, Real>(bp, kp, (Real)ab.x, (Real)ab.y, (Real)cd.x,
+                                                                  args.bt, fac, g);
+                }
+            } else {
+                int i = ket.K > 0 ? part / ket.K : bra.K, k = ket.K > 0 ? part - i * ket.K : 0;
+                while (i < bra.K) {
+                    const PrimPair<Real> bp = prims[bra.pp_start + i];
+                    const double2 ab = args.prim_ab[bra.pp_start + i];
+                    for (; k < ket.K; k += split) {
+                        const PrimPair<Real> kp = prims[ket.pp_start + k];
+                        const double2 cd = args.prim_ab[ket.pp_start + k];
+                        accumulate_grad_quartet<LA, LB, LC, LD, Real>(bp, kp, (Real)ab.x, (Real)ab.y, (Real)cd.x,
+                                                                      args.bt, fac, g);
+                    }
+                    k -= ket.K;
+                    ++i;
+                    while (k >= ket.K) {
+                        k -= ket.K;
+                        ++i;
+                    }
                 }
             }
         }
@@ -463,8 +490,13 @@ hipError_t energy_weighted_density(int n, int nocc, const double *C, const doubl
 }
 
 template <typename Real>
-static hipError_t launch_grad_eri_prec(int cls, const GradEriArgs &a, hipStream_t s) {
-    const dim3 grid((a.ntasks + 3) / 4), block(256);
+static hipError_t launch_grad_eri_prec(int cls, const GradEriArgs &a_in, hipStream_t s) {
+    // split tasks over waves until the launch holds about 4096 waves (two per SIMD of the chip)
+    GradEriArgs a = a_in;
+    a.split = a.ntasks >= 4096 ? 1 : (4096 + a.ntasks - 1) / a.ntasks;
+    if (a.split > 16) a.split = 16;
+    const size_t waves = (size_t)a.ntasks * a.split;
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     const size_t lds = (size_t)3 * a.natoms * sizeof(double);
     switch (cls) {
         case 0: hipLaunchKernelGGL((grad_eri_kernel<0, 0, 0, 0, Real>), grid, block, lds, s, a); break;

@@ -229,8 +229,9 @@ struct GradEriArgs {
     int nbf, natoms;
     BoysTables bt;
     double *grad;  // [natoms][3], accumulated
+    int split;     // waves per task, each taking every split-th primitive quartet (set by the launcher)
 };
-// nuclear gradient, one-electron part: one thread per shell pair of one pair class
+// nuclear gradient, one-electron part: one thread per (shell pair, term) of one pair class
 struct GradOneArgs {
     int nbf, natoms, npairs;
     const int *pair_a, *pair_b, *shell_f0, *shell_atom;
