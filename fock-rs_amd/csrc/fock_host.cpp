@@ -2155,10 +2155,12 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             b.dotm.push_back(nd);
             b.err = hd[16];
             // Fx = the DIIS extrapolation (:202-207) or the newest Fock matrix, as one linear
-            // combination of ring slots for every molecule
+            // combination of ring slots for every molecule; the newest one on the iteration that
+            // returns (accurate policy), as in the single-molecule SCF
             std::vector<double> c{1.0};
             std::vector<int> sl{slot};
-            if (mm >= opts.diis_start) {
+            const bool exit_plain = opts.integral_policy == FOCK_POLICY_ACCURATE && b.err < opts.epsilon;
+            if (mm >= opts.diis_start && !exit_plain) {
                 std::vector<double> Bm((size_t)mm * mm), cd;
                 for (int i = 0; i < mm; ++i)
                     for (int j = 0; j < mm; ++j) Bm[(size_t)i * mm + j] = b.dotm[i][j];
@@ -2663,9 +2665,16 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         const double err = out->last_error;
         for (int i = 0; i < m - 1; ++i) dots[i].push_back(newdots[i]);
         dots.push_back(newdots);
-        // extrapolate from diis_start stored matrices on, the reference's 5 (:202-207)
+        // extrapolate from diis_start stored matrices on, the reference's 5 (:202-207). Accurate
+        // policy: the iteration that returns diagonalises the newest Fock matrix itself, which is
+        // self-consistent there. Its DIIS system can be singular: when the stored errors are all
+        // parallel (H2 in a minimal basis, whose orthogonalised error matrix has one independent
+        // element) the coefficients, and so the returned orbitals and energy (measured: up to
+        // 7e-6 Eh), follow the rounding of the last Fock build.
         const double *Fuse = f_slot;
-        if (m >= opts.diis_start) {
+        const bool exit_plain = opts.integral_policy == FOCK_POLICY_ACCURATE && phase == 1 && !incr_build &&
+                                err < opts.epsilon;
+        if (m >= opts.diis_start && !exit_plain) {
             std::vector<double> Bm((size_t)m * m), c;
             for (int i = 0; i < m; ++i)
                 for (int j = 0; j < m; ++j) Bm[(size_t)i * m + j] = dots[i][j];

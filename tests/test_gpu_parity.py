@@ -194,6 +194,24 @@ def test_scf_small(ctx, small, key, policy, screen, tol):
     assert np.abs(r.density_matrix - small[key + "_P"]).max() < 1e-6
 
 
+def test_scf_h2_singular_diis_repeatable(ctx, small):
+    """H2/STO-3G: every stored DIIS error is a multiple of one matrix (the orthogonalised 2x2
+    error has one independent element), so the DIIS system is singular and its coefficients
+    follow the rounding of the Fock build. The accurate policy's returned iteration diagonalises
+    the newest Fock matrix itself: every repeat gives the fixture's energy, orbital energies and
+    density."""
+    say("scf h2 singular diis")
+    from fock_amd.scf import try_scf
+    b = m.BasisSet.load("STO-3G")
+    atoms = [b.get((0, 0, 0), 1), b.get((0, 0, 1.4), 1)]
+    for rep in range(6):
+        r = try_scf(atoms, 400, 1e-12, 0, policy=abi.POLICY_ACCURATE, screen=abi.SCREEN_NONE, ctx=ctx)
+        assert r is not None and r.converged
+        assert abs(r.total_energy - small["h2_acc_energy"][0]) < 1e-8, (rep, r.total_energy)
+        assert np.abs(r.orbital_energies - small["h2_acc_eps"]).max() < 1e-6, (rep, r.orbital_energies)
+        assert np.abs(r.density_matrix - small["h2_acc_P"]).max() < 1e-6
+
+
 def test_scf_o108_ref_compat_trajectory(ctx, o108):
     """The reference's SCF on n-octane is a chaotic map for its first ~40 iterations (energies
     swing between -236 and -459 Eh), so last-bit differences decide which stationary point it
