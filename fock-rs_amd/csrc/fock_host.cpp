@@ -116,7 +116,31 @@ struct Prepared {
     int64_t max_slots = 0;
     bool compact_ready = false;
     int64_t last_survivors = 0;
+    // the basis (copied) and options this state was prepared for: a gradient of the molecule the
+    // last SCF prepared reuses the pair and task lists
+    std::vector<double> key_d;
+    std::vector<int32_t> key_i;
 };
+
+// Key of a preparation: every array of the basis and the options. False for a descriptor that
+// the preparation itself would reject first (the caller then prepares and reports the error).
+bool basis_key(const fock_basis_desc *b, int32_t policy, int32_t screen, double tau, int32_t precision,
+               std::vector<double> &kd, std::vector<int32_t> &ki) {
+    if (!b || b->n_functions <= 0 || !b->centers || !b->angular || !b->prim_offset || !b->exponents ||
+        !b->coefficients)
+        return false;
+    const size_t n = (size_t)b->n_functions;
+    const int32_t np = b->prim_offset[n];
+    if (np <= 0) return false;
+    kd.assign(b->centers, b->centers + 3 * n);
+    kd.insert(kd.end(), b->exponents, b->exponents + np);
+    kd.insert(kd.end(), b->coefficients, b->coefficients + np);
+    kd.push_back(tau);
+    ki.assign({b->n_functions, policy, screen, precision});
+    ki.insert(ki.end(), b->angular, b->angular + 3 * n);
+    ki.insert(ki.end(), b->prim_offset, b->prim_offset + n + 1);
+    return true;
+}
 
 }  // namespace
 
@@ -922,6 +946,10 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     mark("uploads");
     pr.prim_counted = !pr.prim_screen;  // kept primitive quartets: counted on the first stats request
     HIPCHK(hipStreamSynchronize(S(ctx)));
+    if (!basis_key(basis, policy, screen, tau, precision, pr.key_d, pr.key_i)) {
+        pr.key_d.clear();
+        pr.key_i.clear();
+    }
     pr.valid = true;
     return FOCK_OK;
 }
@@ -2285,7 +2313,15 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
     if ((size_t)3 * na * sizeof(double) > ctx->max_lds) return ctx->fail(FOCK_UNSUPPORTED, "too many atoms");
     HIPCHK(hipSetDevice(ctx->device));
     const bool mixed = opts.precision == FOCK_PRECISION_MIXED;
-    STCHK(fock_jk_prepare(ctx, basis, FOCK_POLICY_ACCURATE, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0));
+    {  // the SCF of the same basis and options (the usual caller) has prepared the task lists
+        std::vector<double> kd;
+        std::vector<int32_t> ki;
+        const bool keyed = basis_key(basis, FOCK_POLICY_ACCURATE, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0,
+                                     kd, ki);
+        if (!(keyed && ctx->jk.valid && !ctx->jk.key_i.empty() && ctx->jk.key_i == ki && ctx->jk.key_d == kd))
+            STCHK(fock_jk_prepare(ctx, basis, FOCK_POLICY_ACCURATE, opts.screen, opts.schwarz_threshold,
+                                  mixed ? 1 : 0));
+    }
     Prepared &pr = ctx->jk;
     const int ns = (int)pr.shells.size();
     // the nucleus each shell sits on (derivatives move basis functions with their nucleus)

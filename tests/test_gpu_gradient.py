@@ -86,6 +86,24 @@ def test_gradient_octane_vs_gpu_fd(ctx):
     assert np.abs(torque(fn, g)).max() < 1e-5
 
 
+def test_gradient_reuses_or_rebuilds_prepared_lists(ctx):
+    """The gradient reuses the task lists the SCF prepared when basis and options are the same,
+    and prepares again otherwise: A after the SCF of A, B (A displaced) after the SCF of B, then A
+    again (lists of B held) and A at another Schwarz threshold give A's gradient."""
+    fa, na = m.flatten(m.build(m.water_cluster_geometry(2, np.random.default_rng(9)), "6-31G"))
+    fbb, nb = m.displaced(fa, na, 1, 2, 0.05)
+    o = opts(screen=abi.SCREEN_SCHWARZ, tau=1e-12, eps=1e-12)
+    ra = energy(ctx, fa, na, o)
+    ga = ctx.rhf_gradient(fa, na, o, ra)
+    rb = energy(ctx, fbb, nb, o)
+    gb = ctx.rhf_gradient(fbb, nb, o, rb)
+    ga2 = ctx.rhf_gradient(fa, na, o, ra)
+    ga3 = ctx.rhf_gradient(fa, na, opts(screen=abi.SCREEN_SCHWARZ, tau=1e-14, eps=1e-12), ra)
+    assert np.abs(gb - ga).max() > 1e-3
+    assert np.abs(ga2 - ga).max() < 1e-12
+    assert np.abs(ga3 - ga).max() < 1e-9
+
+
 def test_gradient_rejects_reference_policy(ctx):
     fb, fn = m.flatten(m.h2o_sto3g())
     o = opts()
