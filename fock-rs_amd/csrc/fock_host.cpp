@@ -46,6 +46,10 @@ constexpr float PRIM_THRESHOLD = 1e-15f;
 // rounding error of ~1e-7 of the quartet's bound, so primitive quartets bounded by 1e-11 are far
 // below it (the result of the mixed form always comes from an FP64 build)
 constexpr float PRIM_THRESHOLD32 = 1e-11f;
+// the screen of the gradient's primitive quartets: a derivative integral exceeds the bound of the
+// integral by up to ~2 alpha (alpha to ~3e3 in the s/p basis sets here), so the cut is 1e3 below
+// the energy's
+constexpr float PRIM_THRESHOLD_GRAD = 1e-18f;
 // mixed-precision gradient: derivative integrals of tasks whose Schwarz bound Q_ab Q_cd (largest
 // ket of the task) is below this run in FP32, the others in FP64
 constexpr float GRAD_TAU64 = 1e-4f;
@@ -2391,6 +2395,10 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
         g.natoms = na;
         g.bt = ctx->bt;
         g.grad = dG.as<double>();
+        if (pr.prim_screen) {  // primitive pairs sorted by q (accurate policy)
+            g.prim_q = pr.d_prim_q.as<float>();
+            g.prim_thr = PRIM_THRESHOLD_GRAD;
+        }
         HIPCHK(launch_grad_eri(cls, g, S(ctx)));
     }
     std::vector<double> gh(3 * na);

@@ -268,6 +268,8 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
                 const int nq = bra.K * ket.K;
                 for (int ik = part; ik < nq; ik += split) {
                     const int i = ik / ket.K, k = ik - i * ket.K;
+                    if (args.prim_q && args.prim_q[bra.pp_start + i] * args.prim_q[ket.pp_start + k] <= args.prim_thr)
+                        continue;
                     const PrimPair<Real> bp = prims[bra.pp_start + i];
                     const double2 ab = args.prim_ab[bra.pp_start + i];
                     const PrimPair<Real> kp = prims[ket.pp_start + k];
@@ -280,7 +282,9 @@ __global__ __launch_bounds__(256) void grad_eri_kernel(GradEriArgs args) {
                 while (i < bra.K) {
                     const PrimPair<Real> bp = prims[bra.pp_start + i];
                     const double2 ab = args.prim_ab[bra.pp_start + i];
+                    const float qb = args.prim_q ? args.prim_q[bra.pp_start + i] : 1.0f;
                     for (; k < ket.K; k += split) {
+                        if (args.prim_q && qb * args.prim_q[ket.pp_start + k] <= args.prim_thr) continue;
                         const PrimPair<Real> kp = prims[ket.pp_start + k];
                         const double2 cd = args.prim_ab[ket.pp_start + k];
                         accumulate_grad_quartet<LA, LB, LC, LD, Real>(bp, kp, (Real)ab.x, (Real)ab.y, (Real)cd.x,

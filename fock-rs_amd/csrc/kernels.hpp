@@ -230,6 +230,10 @@ struct GradEriArgs {
     BoysTables bt;
     double *grad;  // [natoms][3], accumulated
     int split;     // waves per task, each taking every split-th primitive quartet (set by the launcher)
+    // primitive screen: the primitive quartet (ij|kl) is skipped when q_ij q_kl <= prim_thr (q as in
+    // JKArgs::prim_q); null: every primitive quartet is evaluated
+    const float *prim_q;
+    float prim_thr;
 };
 // nuclear gradient, one-electron part: one thread per (shell pair, term) of one pair class
 struct GradOneArgs {
