@@ -192,7 +192,9 @@ fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fock_basis_de
  * reference Boys table is not differentiable); opts->screen / schwarz_threshold select the
  * quartets as in the SCF; opts->precision = FOCK_PRECISION_MIXED evaluates the derivative
  * integrals of quartets with Schwarz bound below 1e-4 in FP32 (FP64 accumulation). Every
- * basis-function centre must coincide with a nucleus. */
+ * basis-function centre must coincide with a nucleus. The context's prepared J/K state is the
+ * one for this basis and options afterwards: kept when the last preparation (fock_jk_prepare or
+ * fock_scf_run) had the same basis arrays and options, prepared again otherwise. */
 fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *basis, const fock_nuclei_desc *nuclei,
                               const fock_scf_opts *opts, const fock_scf_out *scf, double *gradient);
 
