@@ -161,6 +161,58 @@ def batch_form(ctx, abi, m, np):
     }
 
 
+def mo_grid_form(ctx, abi, m, np, torch, fb, C, nocc, cpu=True):
+    """The renderers' loop (volumetric/src/main.rs:100-125): orbitals of the converged C20H42/6-31G
+    SCF on the renderer's grid (200^3 voxels over a 20 Bohr box), FP64 arithmetic, float32 texture
+    written to HBM (fock_mo_grid_device). Timed: the HOMO alone and all occupied orbitals in one
+    call. Algorithmic flop per voxel, counted from the truncated orbital (|c| > 0.05): per shell
+    with kept functions 8 (|d|^2), per primitive 2 (alpha d^2, exp counted as one), per kept
+    function 2K + L + 2 (radial sum, polynomial, accumulation)."""
+    N, box = 200, 20.0
+    step = box / N
+    origin, steps, dims = (-0.5 * box,) * 3, (step,) * 3, (N,) * 3
+    nvox = N ** 3
+    stream = torch.cuda.current_stream()
+
+    def flop_per_voxel(k):
+        shells, f = {}, 0.0
+        for i in range(fb.n):
+            if abs(C[i, k]) > 0.05:
+                key = tuple(fb.centers[i]) + (fb.prim_offset[i + 1] - fb.prim_offset[i], float(fb.exponents[fb.prim_offset[i]]))
+                shells[key] = fb.prim_offset[i + 1] - fb.prim_offset[i]
+                f += 2 * shells[key] + int(fb.angular[i].sum()) + 2
+        return f + sum(8 + 2 * K for K in shells.values())
+
+    out = {}
+    for name, orbs in (("homo", [nocc - 1]), ("occupied", list(range(nocc)))):
+        buf = torch.empty(len(orbs) * nvox, dtype=torch.float32, device="cuda:0")
+        ctx.mo_grid_device(fb, C, orbs, origin, steps, dims, buf.data_ptr())
+        torch.cuda.synchronize()
+        reps = 5 if name == "homo" else 2
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.mo_grid_device(fb, C, orbs, origin, steps, dims, buf.data_ptr())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flop = sum(flop_per_voxel(k) for k in orbs) * nvox
+        out[name] = {"orbitals": len(orbs), "ms": ms, "voxels_per_s": len(orbs) * nvox / (ms * 1e-3),
+                     "achieved_tflops": flop / (ms * 1e-3) / 1e12, "frac_fp64": flop / (ms * 1e-3) / 1e12 / PEAK_FP64}
+        del buf
+    if cpu:  # the oracle's numpy evaluation (one thread) on 200k voxels of the HOMO
+        from oracle import refmo
+        pts = refmo.volumetric_points(N, box)[::40][:200000]
+        t0 = time.time()
+        refmo.evaluate(fb, C, nocc - 1, pts)
+        t = time.time() - t0
+        out["cpu_baseline"] = {"voxels_per_s": len(pts) / t, "cores": 1, "kind": "port",
+                               "sample": "%d voxels of the HOMO through oracle/refmo.py (numpy)" % len(pts)}
+        out["speedup_homo_vs_cpu"] = out["homo"]["voxels_per_s"] / out["cpu_baseline"]["voxels_per_s"]
+    out["workload"] = "C20H42/6-31G orbitals on the volumetric renderer's grid: 200^3 voxels, 20 Bohr box, float32"
+    return out
+
+
 def gradient_form(ctx, abi, m, np):
     """Analytic RHF gradient of n-octane/6-31G at its converged FP64 SCF."""
     fb, fn = m.flatten(m.octane_631g())
@@ -241,6 +293,7 @@ def main():
     e_oracle = float(np.load(gold_path)["e264_acc_energy"][0]) if os.path.exists(gold_path) else None
     scf = {}
     P_ref = None
+    C_fp64 = None
     if not args.no_scf:  # untimed warm-up of both forms (code objects, solver workspaces)
         for prec, incr in ((abi.PRECISION_FP64, 0), (abi.PRECISION_MIXED, 1)):
             ctx.scf_run(fb, fn, abi.default_opts(max_iters=2, epsilon=SCF_EPS, integral_policy=abi.POLICY_ACCURATE,
@@ -263,15 +316,40 @@ def main():
             "status": abi.STATUS_NAMES[st], "total_energy": r["total_energy"], "iterations": r["iterations"],
             "scf_wall_s": r["scf_wall_ms"] / 1e3,
             "fock_build_ms_mean": float(np.mean([h[2] for h in r["history"]])) if r["history"] else None,
+            "fock_build_ms_total": float(np.sum([h[2] for h in r["history"]])) if r["history"] else None,
+            # per iteration: build ms and the shell quartets it evaluated
+            "builds": [[round(h[2], 3), int(h[3])] for h in r["history"]],
         }
         if e_oracle is not None:
             scf[name]["abs_dE_vs_fp64_oracle"] = abs(r["total_energy"] - e_oracle)
         if name == "fp64":
             P_ref = r["density"]
-        log("SCF %s: %s" % (name, scf[name]))
+            C_fp64 = r["coefficients"]
+            nocc = r["n_electrons"] // 2
+        log("SCF %s: %s" % (name, {k: v for k, v in scf[name].items() if k != "builds"}))
     if P_ref is None:  # synthetic density (same screened quartet set: the build does not screen on P)
         P_ref = np.eye(n) * 0.1
 
+    # accuracy of the full builds at the FP64 oracle's converged density P*: max |dF_ij| =
+    # max |d(J - K/2)_ij| against the oracle's unscreened J, K (tests/golden/e264.npz)
+    dF = {}
+    if os.path.exists(gold_path):
+        g = np.load(gold_path)
+        if "e264_acc_P_triu" in g.files:
+            iu = np.triu_indices(n)
+
+            def full(key):
+                A = np.zeros((n, n))
+                A[iu] = g[key]
+                return A + np.triu(A, 1).T
+
+            Pstar, Jstar, Kstar = full("e264_acc_P_triu"), full("e264_acc_J_triu"), full("e264_acc_K_triu")
+            for name, prec in (("fp64", abi.PRECISION_FP64), ("fp32", abi.PRECISION_MIXED)):
+                ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12, prec)
+                J, K = ctx.jk_build(Pstar)
+                dF["max_abs_dF_" + name] = float(np.abs((J - 0.5 * K) - (Jstar - 0.5 * Kstar)).max())
+            P_ref = Pstar
+            log("full-build accuracy at the oracle density: %s" % dF)
     dP = torch.from_numpy(np.ascontiguousarray(P_ref)).to("cuda:0")
     dJ = torch.empty((n, n), dtype=torch.float64, device="cuda:0")
     dK = torch.empty_like(dJ)
@@ -320,11 +398,21 @@ def main():
         "scf": scf, "e_oracle": e_oracle,
         "roofline": roofline,
     }
+    result.update(dF)
+    if "fp64" in scf and "mixed" in scf and scf["mixed"].get("fock_build_ms_mean"):
+        # the SCF-level ratio next to the full-build ratio: mean build of the FP64 SCF over the mean
+        # build of the mixed SCF (FP32 bulk + incremental builds + FP64 cleanup builds)
+        result["scf_build_ratio_fp64_over_mixed"] = scf["fp64"]["fock_build_ms_mean"] / scf["mixed"]["fock_build_ms_mean"]
+        result["scf_mixed_mean_build_over_full_fp32"] = scf["mixed"]["fock_build_ms_mean"] / mixed["ms"]
     if not args.no_scf:
         result["batch_form"] = batch_form(ctx, abi, m, np)
         log("batch form: %s" % result["batch_form"])
         result["gradient_form"] = gradient_form(ctx, abi, m, np)
         log("gradient form: %s" % result["gradient_form"])
+        if C_fp64 is not None:
+            result["mo_grid_form"] = mo_grid_form(ctx, abi, m, np, torch, fb, C_fp64, nocc,
+                                                  cpu=not args.no_cpu_baseline)
+            log("mo grid form: %s" % result["mo_grid_form"])
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
         cb = cpu_baseline(fb, threads=min(16, os.cpu_count() or 1))

@@ -423,10 +423,14 @@ __global__ __launch_bounds__(256) void prim_diag_kernel(PrimDiagArgs args) {
 // Number of primitive quartets the primitive screen keeps over a task list.
 __global__ __launch_bounds__(256) void prim_count_kernel(PrimCountArgs args) {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    unsigned long long c = 0;
+    unsigned long long c = 0, cs = 0;
     if (t < args.ntasks) {
         const Task task = args.tasks[t];
-        if (lane < task.ket_count) {
+        // whole shell quartets below ket_tau are dropped by the kernels that apply it (FP32)
+        bool kept = lane < task.ket_count;
+        if (kept && args.ket_tau > 0.0f) kept = args.pairQ[task.bra] * args.pairQ[task.ket_begin + lane] >= args.ket_tau;
+        if (kept) {
+            cs = 1;
             const PairInfo bra = args.pairs[task.bra], ket = args.pairs[task.ket_begin + lane];
             for (int k = 0; k < ket.K; ++k) {
                 const float qk = args.prim_q[ket.pp_start + k];
@@ -438,7 +442,9 @@ __global__ __launch_bounds__(256) void prim_count_kernel(PrimCountArgs args) {
         }
     }
     c = wave_sum(c);
+    cs = wave_sum(cs);
     if (lane == 0 && c) atomicAdd(args.out, c);
+    if (lane == 0 && cs) atomicAdd(args.out_shell, cs);
 }
 
 // Blocked J/K digestion: one workgroup per block of tasks that share a bra pair (a, b). Every K

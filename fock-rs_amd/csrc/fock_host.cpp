@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -46,10 +47,15 @@ constexpr float PRIM_THRESHOLD = 1e-15f;
 // rounding error of ~1e-7 of the quartet's bound, so primitive quartets bounded by 1e-11 are far
 // below it (the result of the mixed form always comes from an FP64 build)
 constexpr float PRIM_THRESHOLD32 = 1e-11f;
-// the screen of the gradient's primitive quartets: a derivative integral exceeds the bound of the
-// integral by up to ~2 alpha (alpha to ~3e3 in the s/p basis sets here), so the cut is 1e3 below
-// the energy's
-constexpr float PRIM_THRESHOLD_GRAD = 1e-18f;
+// the screen of the gradient's primitive quartets: d/dA_x (ab|cd) = 2 alpha (a+1_x b|cd) - a_x (a-1_x
+// b|cd) exceeds the bound of (ab|cd) by at most about 2 alpha_max (the largest exponent of the basis:
+// 6.1e3 for C in 6-31G, 2.8e4 for Ne in 6-311G), so the cut is the energy's divided by that factor,
+// never looser than 1e-18
+double prim_threshold_grad(const fock_basis_desc *b) {
+    double amax = 0.0;
+    for (int32_t k = 0; k < b->prim_offset[b->n_functions]; ++k) amax = std::max(amax, b->exponents[k]);
+    return (double)PRIM_THRESHOLD / std::max(1e3, 2.0 * amax);
+}
 // mixed-precision gradient: derivative integrals of tasks whose Schwarz bound Q_ab Q_cd (largest
 // ket of the task) is below this run in FP32, the others in FP64
 constexpr float GRAD_TAU64 = 1e-4f;
@@ -124,7 +130,18 @@ struct Prepared {
     // last SCF prepared reuses the pair and task lists
     std::vector<double> key_d;
     std::vector<int32_t> key_i;
+    uint64_t pair_hash = 0;  // FNV-1a of the sorted pair records and primitive-pair data
 };
+
+// FNV-1a over bytes, continuing from h.
+uint64_t fnv1a(uint64_t h, const void *data, size_t bytes) {
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < bytes; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
 
 // Key of a preparation: every array of the basis and the options. False for a descriptor that
 // the preparation itself would reject first (the caller then prepares and reports the error).
@@ -156,6 +173,7 @@ struct fock_ctx {
     rocblas_handle blas = nullptr;
     std::string err;
     DevBuf lut, tab64, tab32, pd64, pd32;
+    DevBuf mo_shells, mo_funs, mo_dbl, mo_off, mo_pts;  // records of the last orbital evaluation
     BoysTables bt{};
     Prepared jk;
     // tasks per block of the blocked digestion per quartet class: [FOCK_TILES_FP64] one molecule,
@@ -166,6 +184,7 @@ struct fock_ctx {
     bool timing = false;
     hipEvent_t tev[NUM_CLASSES + 1] = {};
     std::mutex err_mu;  // preparation threads of a batch may fail at once
+    std::atomic<int64_t> preparations{0};  // task-list preparations made by this context
     fock_status fail(fock_status st, const std::string &msg) {
         std::lock_guard<std::mutex> lock(err_mu);
         err = msg;
@@ -258,6 +277,18 @@ int cart_h(int L, int c, int axis) {
 
 int pair_class_of(int La, int Lb) { return La == 0 ? 0 : (Lb == 0 ? 1 : 2); }
 
+// Point charges: present, with finite positions and charges.
+fock_status check_nuclei(fock_ctx *ctx, const fock_nuclei_desc *nu) {
+    if (!nu || nu->n_atoms <= 0 || !nu->positions || !nu->charges) return ctx->fail(FOCK_INVALID_ARG, "no nuclei");
+    for (int a = 0; a < nu->n_atoms; ++a) {
+        if (!std::isfinite(nu->charges[a])) return ctx->fail(FOCK_INVALID_ARG, "non-finite nuclear charge");
+        for (int x = 0; x < 3; ++x)
+            if (!std::isfinite(nu->positions[3 * a + x]))
+                return ctx->fail(FOCK_INVALID_ARG, "nucleus " + std::to_string(a) + " has a non-finite position");
+    }
+    return FOCK_OK;
+}
+
 // Group the flat function list into shells (basis_set/src/lib.rs:85-113 expands every shell
 // into ncart(L) consecutive functions with shared centre, exponents and coefficients).
 fock_status build_shells(fock_ctx *ctx, const fock_basis_desc *b, std::vector<Shell> &out) {
@@ -287,15 +318,19 @@ fock_status build_shells(fock_ctx *ctx, const fock_basis_desc *b, std::vector<Sh
                                  "function " + std::to_string(g) + " breaks the Cartesian component order of its shell");
             if (b->prim_offset[g + 1] - b->prim_offset[g] != K)
                 return ctx->fail(FOCK_INVALID_ARG, "shell components with different contraction lengths");
-            // bitwise comparison: a NaN centre is carried through (the reference then returns None)
+            // non-finite input is rejected before anything is launched (the reference would carry a
+            // NaN through every integral and return None)
+            for (int x = 0; x < 3; ++x)
+                if (!std::isfinite(b->centers[3 * g + x]))
+                    return ctx->fail(FOCK_INVALID_ARG, "function " + std::to_string(g) + " has a non-finite centre");
             if (std::memcmp(b->centers + 3 * g, b->centers + 3 * f, 3 * sizeof(double)) != 0)
                 return ctx->fail(FOCK_INVALID_ARG, "shell components on different centres");
             for (int k = 0; k < K; ++k) {
-                if (b->exponents[b->prim_offset[g] + k] != b->exponents[b->prim_offset[f] + k] ||
-                    b->coefficients[b->prim_offset[g] + k] != b->coefficients[b->prim_offset[f] + k])
+                const double e = b->exponents[b->prim_offset[g] + k], c = b->coefficients[b->prim_offset[g] + k];
+                if (e != b->exponents[b->prim_offset[f] + k] || c != b->coefficients[b->prim_offset[f] + k])
                     return ctx->fail(FOCK_INVALID_ARG, "shell components with different primitives");
-                if (!(b->exponents[b->prim_offset[g] + k] > 0.0))
-                    return ctx->fail(FOCK_INVALID_ARG, "non-positive exponent");
+                if (!(e > 0.0) || !std::isfinite(e)) return ctx->fail(FOCK_INVALID_ARG, "non-positive or non-finite exponent");
+                if (!std::isfinite(c)) return ctx->fail(FOCK_INVALID_ARG, "non-finite contraction coefficient");
             }
         }
         Shell s;
@@ -555,6 +590,7 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         t_mark = now;
     };
     pr.valid = false;
+    ++ctx->preparations;
     STCHK(build_shells(ctx, basis, pr.shells));
     const int n = basis->n_functions;
     const int ns = (int)pr.shells.size();
@@ -745,6 +781,9 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         rank_sorted[i] = rank_in_class[order[i]];
     }
     HIPCHK(hipMemcpyAsync(pr.d_pairs.p, info.data(), npair * sizeof(PairInfo), hipMemcpyHostToDevice, S(ctx)));
+    // checksum of the screen lists' inputs: pair order and the (q-sorted) primitive-pair data
+    pr.pair_hash = fnv1a(fnv1a(1469598103934665603ull, info.data(), info.size() * sizeof(PairInfo)), prims.data(),
+                         prims.size() * sizeof(PrimPair<double>));
 
     // buckets: contiguous runs of equal (class, K)
     struct Bucket {
@@ -958,27 +997,34 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     return FOCK_OK;
 }
 
-// Primitive quartets the primitive screen keeps, per class (statistics and the algorithmic flop
-// count of the accurate policy), at the threshold of the prepared precision's full builds (FP32
-// kernels for the mixed form); one pass over the task lists.
+// Work the screens keep, per class (statistics and the algorithmic flop count of the accurate
+// policy), at the thresholds of the prepared precision's full builds: shell quartets (the FP32
+// kernels of the mixed form also drop whole shell quartets with Q_ab Q_cd below PRIM_THRESHOLD32),
+// their contracted integrals and their primitive quartets (primitive screen); one pass over the task
+// lists.
 fock_status count_kept_primitives(fock_ctx *ctx) {
     Prepared &pr = ctx->jk;
     if (pr.prim_counted) return FOCK_OK;
     {
-        HIPCHK(pr.d_count.ensure(NUM_CLASSES * sizeof(unsigned long long)));
-        HIPCHK(fill_f64(NUM_CLASSES, 0.0, pr.d_count.as<double>(), S(ctx)));  // all-zero bits
+        const bool fp32 = pr.precision == 1;
+        HIPCHK(pr.d_count.ensure(2 * NUM_CLASSES * sizeof(unsigned long long)));
+        HIPCHK(fill_f64(2 * NUM_CLASSES, 0.0, pr.d_count.as<double>(), S(ctx)));  // all-zero bits
         for (int cls = 0; cls < NUM_CLASSES; ++cls) {
             PrimCountArgs ca{pr.d_tasks.as<Task>() + pr.task_off[cls], pr.task_off[cls + 1] - pr.task_off[cls],
                              pr.d_pairs.as<PairInfo>(), pr.d_prim_q.as<float>(),
-                             pr.precision == 1 ? PRIM_THRESHOLD32 : PRIM_THRESHOLD,
-                             pr.d_count.as<unsigned long long>() + cls};
+                             fp32 ? PRIM_THRESHOLD32 : PRIM_THRESHOLD,
+                             fp32 ? pr.d_pairQ.as<float>() : nullptr, fp32 ? PRIM_THRESHOLD32 : 0.0f,
+                             pr.d_count.as<unsigned long long>() + cls,
+                             pr.d_count.as<unsigned long long>() + NUM_CLASSES + cls};
             HIPCHK(launch_prim_count(ca, S(ctx)));
         }
-        unsigned long long kept[NUM_CLASSES];
+        unsigned long long kept[2 * NUM_CLASSES];
         HIPCHK(hipMemcpyAsync(kept, pr.d_count.p, sizeof(kept), hipMemcpyDeviceToHost, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
         for (int cls = 0; cls < NUM_CLASSES; ++cls) {
             pr.stats.primitive_quartets[cls] = (int64_t)kept[cls];
+            pr.stats.shell_quartets[cls] = (int64_t)kept[NUM_CLASSES + cls];
+            pr.stats.integrals[cls] = pr.stats.shell_quartets[cls] * CLASS_NINT[cls];
             pr.stats.algorithmic_flop[cls] =
                 (double)kept[cls] * F_PRIM[cls] + 12.0 * (double)pr.stats.integrals[cls];
         }
@@ -1016,7 +1062,13 @@ extern "C" fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats) {
     if (!ctx->jk.valid) return ctx->fail(FOCK_INVALID_ARG, "fock_jk_prepare has not run");
     HIPCHK(hipSetDevice(ctx->device));
     STCHK(count_kept_primitives(ctx));
-    *stats = ctx->jk.stats;
+    Prepared &pr = ctx->jk;
+    *stats = pr.stats;
+    uint64_t h = pr.pair_hash;
+    h = fnv1a(h, pr.h_tasks.data(), pr.h_tasks.size() * sizeof(Task));
+    h = fnv1a(h, pr.h_blocks.data(), pr.h_blocks.size() * sizeof(Block));
+    stats->list_checksum = h;
+    stats->preparations = ctx->preparations.load();
     return FOCK_OK;
 }
 
@@ -1456,8 +1508,7 @@ extern "C" fock_status fock_one_electron(fock_ctx *ctx, const fock_basis_desc *b
                                          int32_t policy, double *Sm, double *Tm, double *Vm) {
     if (!ctx) return FOCK_INVALID_ARG;
     if (policy != 0 && policy != 1) return ctx->fail(FOCK_INVALID_ARG, "unknown integral policy");
-    if (nuclei && nuclei->n_atoms > 0 && (!nuclei->positions || !nuclei->charges))
-        return ctx->fail(FOCK_INVALID_ARG, "nuclei with null arrays");
+    if (nuclei && nuclei->n_atoms > 0) STCHK(check_nuclei(ctx, nuclei));
     HIPCHK(hipSetDevice(ctx->device));
     std::vector<Shell> shells;
     STCHK(build_shells(ctx, basis, shells));
@@ -1476,6 +1527,167 @@ extern "C" fock_status fock_one_electron(fock_ctx *ctx, const fock_basis_desc *b
     return FOCK_OK;
 }
 
+// ====================================================================== orbitals at points
+
+namespace {
+
+// Records of the orbitals `orbitals` (columns of the column-major n x n C): per orbital, the shells
+// holding kept functions (|C_fk| > 0.05, MolecularOrbitals::new, scf/src/molecular_orbitals.rs:47,
+// 69-90) in basis order, each kept function with its weights C_fk coef_p.
+struct MoHost {
+    std::vector<MoShellRec> shells;
+    std::vector<MoFunRec> funs;
+    std::vector<double> dbl;
+    std::vector<int> shell_off;
+};
+
+constexpr double MO_MIN_COEFFICIENT = 0.05;
+
+fock_status build_mo_records(fock_ctx *ctx, const fock_basis_desc *b, const double *C, const int32_t *orbitals,
+                             int32_t norb, MoHost &h) {
+    std::vector<Shell> shells;
+    STCHK(build_shells(ctx, b, shells));
+    const int n = b->n_functions;
+    for (int o = 0; o < norb; ++o) {
+        const int k = orbitals[o];
+        if (k < 0 || k >= n) return ctx->fail(FOCK_INVALID_ARG, "orbital index out of range");
+        h.shell_off.push_back((int)h.shells.size());
+        const double *col = C + (size_t)k * n;
+        for (const Shell &sh : shells) {
+            int kept[16], nk = 0;
+            for (int c = 0; c < sh.nf; ++c) {
+                const double v = col[sh.f0 + c];
+                if (!std::isfinite(v)) return ctx->fail(FOCK_INVALID_ARG, "non-finite coefficient matrix");
+                if (std::fabs(v) > MO_MIN_COEFFICIENT) kept[nk++] = sh.f0 + c;
+            }
+            if (nk == 0) continue;
+            MoShellRec r{};
+            for (int x = 0; x < 3; ++x) r.A[x] = sh.A[x];
+            r.K = sh.K;
+            r.nf = nk;
+            r.alpha = (int)h.dbl.size();
+            r.fun = (int)h.funs.size();
+            r.amin = HUGE_VAL;
+            for (int p = 0; p < sh.K; ++p) {
+                const double e = b->exponents[sh.prim0 + p];
+                h.dbl.push_back(e);
+                r.amin = std::min(r.amin, e);
+            }
+            for (int i = 0; i < nk; ++i) {
+                const int f = kept[i];
+                MoFunRec fr{};
+                for (int x = 0; x < 3; ++x) fr.ang[x] = b->angular[3 * f + x];
+                fr.w = (int)h.dbl.size();
+                for (int p = 0; p < sh.K; ++p) h.dbl.push_back(col[f] * b->coefficients[b->prim_offset[f] + p]);
+                h.funs.push_back(fr);
+            }
+            h.shells.push_back(r);
+        }
+    }
+    h.shell_off.push_back((int)h.shells.size());
+    if (h.dbl.size() > (size_t)INT32_MAX / 2) return ctx->fail(FOCK_INVALID_ARG, "too many orbital records");
+    return FOCK_OK;
+}
+
+// Upload the records of `orbitals` and launch the evaluation at npts points (grid or host list)
+// into the device array out (float or double, orbital o at out + o * npts). The records are
+// copied synchronously; the launch itself is asynchronous on the context's stream.
+fock_status mo_eval(fock_ctx *ctx, const fock_basis_desc *b, const double *C, const int32_t *orbitals, int32_t norb,
+                    const fock_grid_desc *grid, const double *points, int64_t npts, void *out, bool f32) {
+    if (!b || !C || !orbitals || norb <= 0 || norb > 65535 || !out)
+        return ctx->fail(FOCK_INVALID_ARG, "bad orbital evaluation arguments");
+    MoHost h;
+    STCHK(build_mo_records(ctx, b, C, orbitals, norb, h));
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(ctx->mo_shells.ensure(std::max<size_t>(h.shells.size(), 1) * sizeof(MoShellRec)));
+    HIPCHK(ctx->mo_funs.ensure(std::max<size_t>(h.funs.size(), 1) * sizeof(MoFunRec)));
+    HIPCHK(ctx->mo_dbl.ensure(std::max<size_t>(h.dbl.size(), 1) * sizeof(double)));
+    HIPCHK(ctx->mo_off.ensure(h.shell_off.size() * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(ctx->mo_shells.p, h.shells.data(), h.shells.size() * sizeof(MoShellRec),
+                          hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(ctx->mo_funs.p, h.funs.data(), h.funs.size() * sizeof(MoFunRec), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(ctx->mo_dbl.p, h.dbl.data(), h.dbl.size() * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(hipMemcpyAsync(ctx->mo_off.p, h.shell_off.data(), h.shell_off.size() * sizeof(int), hipMemcpyHostToDevice,
+                          S(ctx)));
+    MoArgs a{};
+    a.shells = ctx->mo_shells.as<MoShellRec>();
+    a.funs = ctx->mo_funs.as<MoFunRec>();
+    a.dbl = ctx->mo_dbl.as<double>();
+    a.shell_off = ctx->mo_off.as<int>();
+    a.norb = norb;
+    a.npoints = npts;
+    a.out = out;
+    a.out_stride = npts;
+    if (grid) {
+        for (int x = 0; x < 3; ++x) {
+            a.origin[x] = grid->origin[x];
+            a.step[x] = grid->step[x];
+            a.dims[x] = grid->dims[x];
+        }
+    } else {
+        HIPCHK(ctx->mo_pts.ensure((size_t)std::max<int64_t>(npts, 1) * 3 * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(ctx->mo_pts.p, points, (size_t)npts * 3 * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+        a.points = ctx->mo_pts.as<double>();
+    }
+    HIPCHK(hipStreamSynchronize(S(ctx)));  // the host records go out of scope
+    HIPCHK(launch_mo_eval(a, grid != nullptr, f32, S(ctx)));
+    return FOCK_OK;
+}
+
+fock_status check_grid(fock_ctx *ctx, const fock_grid_desc *g, int64_t &nvox) {
+    if (!g) return ctx->fail(FOCK_INVALID_ARG, "null grid");
+    nvox = 1;
+    for (int x = 0; x < 3; ++x) {
+        if (g->dims[x] <= 0 || !std::isfinite(g->origin[x]) || !std::isfinite(g->step[x]))
+            return ctx->fail(FOCK_INVALID_ARG, "grid needs positive dimensions and a finite origin and step");
+        nvox *= g->dims[x];
+    }
+    if (nvox > ((int64_t)1 << 40)) return ctx->fail(FOCK_INVALID_ARG, "grid too large");
+    return FOCK_OK;
+}
+
+}  // namespace
+
+extern "C" fock_status fock_mo_grid(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
+                                    const int32_t *orbitals, int32_t n_orbitals, const fock_grid_desc *grid, float *out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    int64_t nvox = 0;
+    STCHK(check_grid(ctx, grid, nvox));
+    if (!out) return ctx->fail(FOCK_INVALID_ARG, "null output");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf d;
+    HIPCHK(d.ensure((size_t)std::max(n_orbitals, 1) * nvox * sizeof(float)));
+    STCHK(mo_eval(ctx, basis, coefficients, orbitals, n_orbitals, grid, nullptr, nvox, d.p, true));
+    HIPCHK(hipMemcpyAsync(out, d.p, (size_t)n_orbitals * nvox * sizeof(float), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
+extern "C" fock_status fock_mo_grid_device(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
+                                           const int32_t *orbitals, int32_t n_orbitals, const fock_grid_desc *grid,
+                                           float *d_out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    int64_t nvox = 0;
+    STCHK(check_grid(ctx, grid, nvox));
+    return mo_eval(ctx, basis, coefficients, orbitals, n_orbitals, grid, nullptr, nvox, d_out, true);
+}
+
+extern "C" fock_status fock_mo_points(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
+                                      int32_t orbital, int64_t n_points, const double *points, double *out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    if (n_points < 0 || (n_points > 0 && (!points || !out))) return ctx->fail(FOCK_INVALID_ARG, "bad points");
+    if (n_points == 0) return FOCK_OK;
+    for (int64_t i = 0; i < 3 * n_points; ++i)
+        if (!std::isfinite(points[i])) return ctx->fail(FOCK_INVALID_ARG, "non-finite point");
+    HIPCHK(hipSetDevice(ctx->device));
+    DevBuf d;
+    HIPCHK(d.ensure((size_t)n_points * sizeof(double)));
+    STCHK(mo_eval(ctx, basis, coefficients, &orbital, 1, nullptr, points, n_points, d.p, false));
+    HIPCHK(hipMemcpyAsync(out, d.p, (size_t)n_points * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    return FOCK_OK;
+}
+
 // ====================================================================== SCF
 
 namespace {
@@ -1490,6 +1702,21 @@ fock_status eig_sym(fock_ctx *ctx, int n, double *A, double *w, double *work, in
 }
 
 bool is_normal(double x) { return std::isnormal(x); }
+
+// The Loewdin transform X = U diag(1/sqrt(s)) U^T (scf/src/lib.rs:139-146) needs a positive definite
+// overlap: an eigenvalue s_min <= n eps s_max (coincident centres with equal functions, for
+// example) makes it undefined within rounding, where the reference would go on with 1/sqrt of zero,
+// a negative or a rounding-level number and end in None. w: ascending eigenvalues on the device.
+fock_status overlap_definite(fock_ctx *ctx, int n, const double *w) {
+    double lam[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(&lam[0], w, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipMemcpyAsync(&lam[1], w + n - 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    if (lam[0] > (double)n * DBL_EPSILON * lam[1]) return FOCK_OK;
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "overlap matrix is singular (eigenvalues %.3e .. %.3e): SCF diverged", lam[0], lam[1]);
+    return ctx->fail(FOCK_DIVERGED, msg);
+}
 
 // DIIS coefficients (scf/src/utils.rs:102-138): B c = (0, ..., 0, -1) with B_ij = <e_i, e_j>,
 // bordered by -1. Solved by Gaussian elimination with partial pivoting; false when a pivot
@@ -1695,6 +1922,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         bm->nuc = nuclei + m;
         bm->out = outs + m;
         const fock_nuclei_desc *nu = bm->nuc;
+        STCHK(check_nuclei(ctx, nu));
         double zsum = 0.0;
         for (int a = 0; a < nu->n_atoms; ++a) zsum += nu->charges[a];
         const int ne = (int)std::llround(zsum) - opts.molecule_charge;
@@ -1849,8 +2077,27 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     };
     // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146), eigenproblems batched
     STCHK(eig_all(false));
+    // a molecule whose overlap is singular ends DIVERGED here, as in fock_scf_run
+    int n_singular = 0;
+    {
+        std::vector<double> w;
+        for (auto &gp : gr) {
+            const int gn = gp->n;
+            w.resize(gp->members.size() * (size_t)gn);
+            HIPCHK(hipMemcpyAsync(w.data(), gp->W.p, w.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (size_t i = 0; i < gp->members.size(); ++i)
+                if (!(w[i * gn] > (double)gn * DBL_EPSILON * w[i * gn + gn - 1])) {
+                    BatchMol &b = *mols[gp->members[i]];
+                    b.done = true;
+                    b.status = FOCK_DIVERGED;
+                    ++n_singular;
+                }
+        }
+    }
     for (auto &bp : mols) {
         BatchMol &b = *bp;
+        if (b.done) continue;
         const int n = b.n;
         double *T1 = b.T1.as<double>(), *T2 = b.T2.as<double>(), *U = b.U.as<double>();
         HIPCHK(transpose_f64(n, gA(b), U, st));
@@ -1902,7 +2149,8 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     };
     phase_mark("one_elec_X");
     // GWH guess (scf/src/lib.rs:150-162)
-    for (auto &bp : mols) HIPCHK(gwh_guess(bp->n, bp->S.as<double>(), bp->H.as<double>(), bp->F.as<double>(), st));
+    for (auto &bp : mols)
+        if (!bp->done) HIPCHK(gwh_guess(bp->n, bp->S.as<double>(), bp->H.as<double>(), bp->F.as<double>(), st));
     STCHK(diagonalize_all([](BatchMol &b) { return (const double *)b.F.as<double>(); }));
 
     // batched Fock-build tables: one record per (molecule, class); per class, the blocks of every
@@ -2082,7 +2330,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     for (int c = 0; c < NUM_CLASSES; ++c) STCHK(make_schedule(c));
     phase_mark("guess_tables");
     std::vector<double> hostd((size_t)count * 18);
-    int active = count;
+    int active = count - n_singular;
     // G(P), F = H + G of the whole batch: one launch per quartet class; converged molecules'
     // blocks return at once (their record is inactive). The sequence is the same every iteration
     // (kernel arguments are device pointers), so it is captured once as a HIP graph and replayed;
@@ -2159,6 +2407,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
                                   st));
             HIPCHK(hipStreamSynchronize(st));
         }
+        bool diverged_now = false;
         std::vector<CombRec> comb;
         std::vector<GemmRec> ls1, ls2;
         std::vector<ShiftRec> shift;
@@ -2186,6 +2435,14 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             for (int i = 0; i < mm - 1; ++i) b.dotm[i].push_back(nd[i]);
             b.dotm.push_back(nd);
             b.err = hd[16];
+            if (!is_normal(b.err) && !(b.err < opts.epsilon)) {  // NaN or inf: this molecule stops here
+                b.done = true;
+                b.status = FOCK_DIVERGED;
+                b.iter = iter;
+                --active;
+                diverged_now = true;
+                continue;
+            }
             // Fx = the DIIS extrapolation (:202-207) or the newest Fock matrix, as one linear
             // combination of ring slots for every molecule; the newest one on the iteration that
             // returns (accurate policy), as in the single-molecule SCF
@@ -2253,7 +2510,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
                                   st));
             HIPCHK(hipStreamSynchronize(st));
         }
-        bool changed = false;
+        bool changed = diverged_now;
         for (int m = 0; m < count; ++m) {
             BatchMol &b = *mols[m];
             if (b.done) continue;
@@ -2308,6 +2565,11 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
     else fock_scf_opts_default(&opts);
     if (opts.integral_policy != FOCK_POLICY_ACCURATE)
         return ctx->fail(FOCK_INVALID_ARG, "gradients are defined for the accurate integral policy only");
+    // the reference's 1e-6 basis-function-pair screen zeroes single integrals of a kept shell quartet,
+    // so the energy it gives is not a differentiable function of the nuclear positions
+    if (opts.screen == FOCK_SCREEN_REF_BF_PAIR)
+        return ctx->fail(FOCK_INVALID_ARG, "gradients need the Schwarz screen or none, not the 1e-6 pair screen");
+    STCHK(check_nuclei(ctx, nuclei));
     if (!scf->density || !scf->coefficients || !scf->orbital_energies)
         return ctx->fail(FOCK_INVALID_ARG, "the SCF result must hold density, coefficients and orbital energies");
     const int n = basis->n_functions, na = nuclei->n_atoms;
@@ -2328,12 +2590,15 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
     }
     Prepared &pr = ctx->jk;
     const int ns = (int)pr.shells.size();
+    const float prim_thr_grad = (float)prim_threshold_grad(basis);
     // the nucleus each shell sits on (derivatives move basis functions with their nucleus)
     std::vector<int> atom(ns);
     for (int s = 0; s < ns; ++s) {
         atom[s] = -1;
         for (int a = 0; a < na && atom[s] < 0; ++a)
-            if (std::memcmp(pr.shells[s].A, nuclei->positions + 3 * a, 3 * sizeof(double)) == 0) atom[s] = a;
+            if (pr.shells[s].A[0] == nuclei->positions[3 * a] && pr.shells[s].A[1] == nuclei->positions[3 * a + 1] &&
+                pr.shells[s].A[2] == nuclei->positions[3 * a + 2])  // by value: +0.0 and -0.0 are one position
+                atom[s] = a;
         if (atom[s] < 0) return ctx->fail(FOCK_INVALID_ARG, "a basis function centre is not on a nucleus");
     }
     const size_t bytes = (size_t)n * n * sizeof(double);
@@ -2397,7 +2662,7 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
         g.grad = dG.as<double>();
         if (pr.prim_screen) {  // primitive pairs sorted by q (accurate policy)
             g.prim_q = pr.d_prim_q.as<float>();
-            g.prim_thr = PRIM_THRESHOLD_GRAD;
+            g.prim_thr = prim_thr_grad;
         }
         HIPCHK(launch_grad_eri(cls, g, S(ctx)));
     }
@@ -2426,8 +2691,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     if (opts_in) opts = *opts_in;
     else fock_scf_opts_default(&opts);
     if (opts.max_iters < 0) return ctx->fail(FOCK_INVALID_ARG, "negative max_iters");
-    if (nuclei->n_atoms <= 0 || !nuclei->positions || !nuclei->charges)
-        return ctx->fail(FOCK_INVALID_ARG, "no nuclei");
+    STCHK(check_nuclei(ctx, nuclei));
     if (opts.incremental && opts.precision != FOCK_PRECISION_MIXED)
         return ctx->fail(FOCK_INVALID_ARG, "incremental builds are part of the mixed-precision form");
     if (opts.diis_window < 1 || opts.diis_window > 16 || opts.diis_start < 1)
@@ -2503,6 +2767,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146)
     HIPCHK(copy_f64(nn, S_, T1, S(ctx)));
     STCHK(eig_sym(ctx, n, T1, W_, dWork.as<double>(), dInfo.as<int>()));  // T1 = R (rows = eigenvectors)
+    out->n_electrons = n_electrons;
+    out->n_basis = n;
+    STCHK(overlap_definite(ctx, n, W_));
     HIPCHK(transpose_f64(n, T1, U_, S(ctx)));                               // U: columns = eigenvectors
     HIPCHK(gemm_f64(false, false, n, n, n, 1.0, S_, n, U_, n, 0.0, T2, n, S(ctx)));   // S U
     HIPCHK(gemm_f64(true, false, n, n, n, 1.0, U_, n, T2, n, 0.0, T1, n, S(ctx)));    // U^T S U
@@ -2606,9 +2873,13 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     // error of such a quartet, ~1e-7 of its bound, stays below 1e-11)
     constexpr float INC_TAU_BULK = 1e-9f, INC_TAU = 1e-11f, INC_TAU64 = 1e-4f;
     constexpr double MIXED_SWITCH = 1e-4, MIXED_STALL = 1e-2;
-    int since_rebuild = 0, stall = 0;
+    int since_rebuild = 0, stall = 0, stall1 = 0;
     bool need_full = false;  // next build of phase 1 is an FP64 full build
-    double best_err = HUGE_VAL;
+    // phase 1: when the error stops halving for STALL1 iterations (the increments' screening and
+    // FP32 noise floor lies above epsilon), the remaining iterations take FP64 full builds
+    bool inc_off = false;
+    constexpr int STALL1 = 4;
+    double best_err = HUGE_VAL, best_err1 = HUGE_VAL;
     const bool want_counts = out->history != nullptr;
     int64_t static_quartets = 0;
     for (int c = 0; c < NUM_CLASSES; ++c) static_quartets += pr.stats.shell_quartets[c];
@@ -2641,7 +2912,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         // G(P), F = H + G (scf/src/lib.rs:177-187)
         HIPCHK(hipEventRecord(ev0, S(ctx)));
         int64_t evaluated = static_quartets;
-        const bool incr_build = incremental && (phase == 0 || !need_full);
+        const bool incr_build = incremental && !inc_off && (phase == 0 || !need_full);
         if (incr_build) {
             if (iter == 0) {  // G(0) = 0
                 HIPCHK(fill_f64(nn, 0.0, Gacc, S(ctx)));
@@ -2707,6 +2978,10 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             out->last_error = hd[16];
         }
         const double err = out->last_error;
+        if (!is_normal(err) && !(err < opts.epsilon)) {  // NaN or inf: nothing below is meaningful
+            result = FOCK_DIVERGED;
+            break;
+        }
         for (int i = 0; i < m - 1; ++i) dots[i].push_back(newdots[i]);
         dots.push_back(newdots);
         // extrapolate from diis_start stored matrices on, the reference's 5 (:202-207). Accurate
@@ -2787,6 +3062,15 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 break;
             }
             continue;
+        }
+        if (incremental && !inc_off) {
+            if (err < best_err1 * 0.5) {
+                best_err1 = err;
+                stall1 = 0;
+            } else if (++stall1 >= STALL1) {
+                inc_off = true;
+                need_full = true;
+            }
         }
         if (incr_build && iter < opts.max_iters) {
             // an incremental result is never final: reaching epsilon asks for the FP64 cleanup

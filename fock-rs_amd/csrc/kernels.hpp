@@ -208,7 +208,10 @@ struct PrimCountArgs {
     const PairInfo *pairs;
     const float *prim_q;
     float prim_thr;
-    unsigned long long *out;
+    const float *pairQ;  // shell-quartet cut Q_ab Q_cd >= ket_tau when ket_tau > 0 (FP32 kernels)
+    float ket_tau;
+    unsigned long long *out;        // kept primitive quartets
+    unsigned long long *out_shell;  // kept shell quartets
 };
 
 // nuclear gradient, two-electron part: one lane per unique shell quartet of the task lists
@@ -255,6 +258,35 @@ hipError_t launch_grad_one(int pair_class, const GradOneArgs &a, hipStream_t s);
 hipError_t energy_weighted_density(int n, int nocc, const double *C, const double *eps, double *W, hipStream_t s);
 hipError_t launch_prim_diag(int pair_class, const PrimDiagArgs &a, hipStream_t s);
 hipError_t launch_prim_count(const PrimCountArgs &a, hipStream_t s);
+// mo_grid.hip: orbitals at points (regular grid or a list); the records of every orbital are built
+// on the host from the basis and the coefficient matrix (kept functions grouped by shell)
+constexpr int MO_MAXK = 8;  // most primitives per contracted function
+struct MoShellRec {
+    double A[3];
+    double amin;     // smallest exponent of the shell
+    int K, nf;       // primitives; kept functions of this orbital on the shell
+    int alpha;       // dbl[alpha .. alpha + K): exponents
+    int fun;         // funs[fun .. fun + nf)
+};
+struct MoFunRec {
+    int ang[3];
+    int w;           // dbl[w .. w + K): C_fk * coef_p
+};
+struct MoArgs {
+    const MoShellRec *shells;
+    const MoFunRec *funs;
+    const double *dbl;
+    const int *shell_off;  // orbital o: shells[shell_off[o] .. shell_off[o + 1])
+    int norb;
+    int64_t npoints;
+    double origin[3], step[3];  // grid form: point (x, y, z) = origin + (x, y, z) * step
+    int dims[3];                // grid form: point number x + dims[0] (y + dims[1] z)
+    const double *points;       // list form: [npoints][3]
+    void *out;                  // float or double, out[o * out_stride + v]
+    int64_t out_stride;
+};
+hipError_t launch_mo_eval(const MoArgs &a, bool grid, bool f32, hipStream_t s);
+
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);
 hipError_t launch_eri_blocks(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s);
 // dynamic LDS of one workgroup of the blocked kernel of class cls

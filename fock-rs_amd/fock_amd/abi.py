@@ -48,6 +48,7 @@ EXPORTED = [
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
     "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
     "fock_jk_autotune", "fock_tiles_get", "fock_tiles_set", "fock_sorted_eigs_batch",
+    "fock_mo_grid", "fock_mo_grid_device", "fock_mo_points",
 ]
 
 
@@ -88,7 +89,12 @@ class JKStats(ctypes.Structure):
     _fields_ = [("n_basis", ctypes.c_int32), ("n_shells", ctypes.c_int32), ("n_shell_pairs", ctypes.c_int32),
                 ("shell_quartets", ctypes.c_int64 * 6), ("primitive_quartets", ctypes.c_int64 * 6),
                 ("integrals", ctypes.c_int64 * 6), ("tasks", ctypes.c_int64 * 6),
-                ("algorithmic_flop", ctypes.c_double * 6), ("near_threshold", ctypes.c_int64)]
+                ("algorithmic_flop", ctypes.c_double * 6), ("near_threshold", ctypes.c_int64),
+                ("preparations", ctypes.c_int64), ("list_checksum", ctypes.c_uint64)]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_double * 3), ("step", ctypes.c_double * 3), ("dims", ctypes.c_int32 * 3)]
 
 
 class TuneReport(ctypes.Structure):
@@ -144,6 +150,9 @@ def load(path: Optional[str] = None):
         "fock_jk_autotune": (ctypes.c_int, [vp, _dp, ctypes.POINTER(TuneReport)]),
         "fock_tiles_get": (ctypes.c_int, [vp, i32, _ip, _ip]),
         "fock_tiles_set": (ctypes.c_int, [vp, i32, _ip, i32]),
+        "fock_mo_grid": (ctypes.c_int, [vp, bp, _dp, _ip, i32, ctypes.POINTER(GridDesc), ctypes.POINTER(ctypes.c_float)]),
+        "fock_mo_grid_device": (ctypes.c_int, [vp, bp, _dp, _ip, i32, ctypes.POINTER(GridDesc), vp]),
+        "fock_mo_points": (ctypes.c_int, [vp, bp, _dp, i32, i64, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -274,6 +283,7 @@ class Context:
             "shell_quartets": list(st.shell_quartets), "primitive_quartets": list(st.primitive_quartets),
             "integrals": list(st.integrals), "tasks": list(st.tasks),
             "algorithmic_flop": list(st.algorithmic_flop), "near_threshold": st.near_threshold,
+            "preparations": st.preparations, "list_checksum": st.list_checksum,
         }
 
     def gemm(self, A, B, transA=False, transB=False, alpha=1.0, beta=0.0, C=None):
@@ -374,6 +384,42 @@ class Context:
                 "iterations": o.iterations, "n_electrons": o.n_electrons, "n_basis": o.n_basis,
                 "last_error": o.last_error, "scf_wall_ms": o.scf_wall_ms})
         return st, list(status), results
+
+    # orbitals at points -------------------------------------------------------------
+    @staticmethod
+    def _coef_cm(coefficients) -> np.ndarray:
+        """numpy [i, k] (row i, orbital k) -> the C ABI's column-major storage."""
+        return np.ascontiguousarray(np.asarray(coefficients, dtype=np.float64).T).ravel()
+
+    def mo_grid(self, fb, coefficients, orbitals, origin, step, dims) -> np.ndarray:
+        """Orbitals on a grid, float32 [len(orbitals), dims[2], dims[1], dims[0]] (voxel x fastest)."""
+        orb = np.ascontiguousarray(np.atleast_1d(orbitals), dtype=np.int32)
+        g = GridDesc((ctypes.c_double * 3)(*origin), (ctypes.c_double * 3)(*step), (ctypes.c_int32 * 3)(*dims))
+        out = np.zeros((len(orb), dims[2], dims[1], dims[0]), dtype=np.float32)
+        C = self._coef_cm(coefficients)
+        bd = basis_desc(fb)
+        self.check(self.lib.fock_mo_grid(self.ptr, ctypes.byref(bd), _d(C), _i(orb), len(orb), ctypes.byref(g),
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def mo_grid_device(self, fb, coefficients, orbitals, origin, step, dims, d_out: int):
+        """fock_mo_grid into a device array (e.g. a torch tensor's data_ptr()), asynchronous."""
+        orb = np.ascontiguousarray(np.atleast_1d(orbitals), dtype=np.int32)
+        g = GridDesc((ctypes.c_double * 3)(*origin), (ctypes.c_double * 3)(*step), (ctypes.c_int32 * 3)(*dims))
+        C = self._coef_cm(coefficients)
+        bd = basis_desc(fb)
+        self.check(self.lib.fock_mo_grid_device(self.ptr, ctypes.byref(bd), _d(C), _i(orb), len(orb),
+                                                ctypes.byref(g), ctypes.c_void_p(d_out)))
+
+    def mo_points(self, fb, coefficients, orbital: int, points) -> np.ndarray:
+        """One orbital at points [m, 3] (Bohr), float64."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        out = np.zeros(len(pts))
+        C = self._coef_cm(coefficients)
+        bd = basis_desc(fb)
+        self.check(self.lib.fock_mo_points(self.ptr, ctypes.byref(bd), _d(C), int(orbital), len(pts), _d(pts),
+                                           _d(out)))
+        return out
 
     def rhf_gradient(self, fb, fn, opts: ScfOpts, result) -> np.ndarray:
         """dE/dR (Eh/Bohr, shape (n_atoms, 3)) at a converged scf_run result."""

@@ -53,10 +53,16 @@ class MolecularOrbital:
 
 
 class MolecularOrbitals:
-    """scf/src/molecular_orbitals.rs:46-99: keeps coefficients with |c| > 0.05 per orbital."""
+    """scf/src/molecular_orbitals.rs:46-99: keeps coefficients with |c| > 0.05 per orbital.
+
+    grid() and points() evaluate orbitals on the GPU (fock_mo_grid / fock_mo_points) with the same
+    truncation: the renderers' loops (volumetric/src/main.rs:100-125, pointcloud/src/main.rs:106-121)."""
     MIN_COEFFICIENT_MAGNITUDE = 0.05
 
     def __init__(self, basis_functions: List[ContractedGaussian], coeff_matrix: np.ndarray):
+        self._coefficients = np.ascontiguousarray(coeff_matrix, dtype=np.float64)
+        self._flat = None
+        self._basis_functions = list(basis_functions)
         self.molecular_orbitals = []
         for k in range(coeff_matrix.shape[1]):
             col = coeff_matrix[:, k]
@@ -66,6 +72,23 @@ class MolecularOrbitals:
 
     def __getitem__(self, index: int) -> MolecularOrbital:
         return self.molecular_orbitals[index]
+
+    def _flat_basis(self):
+        if self._flat is None:
+            from .molecule import Atom
+            self._flat, _ = flatten([Atom(np.zeros(3), 0, self._basis_functions)])
+        return self._flat
+
+    def grid(self, orbitals, n_voxels: int = 200, box: float = 20.0, ctx: Optional[abi.Context] = None) -> np.ndarray:
+        """float32 [len(orbitals), N, N, N] (z, y, x) of the renderer's voxel grid: voxel (x, y, z) at
+        ((x, y, z) / N - 0.5) * box (volumetric/src/main.rs:111-116)."""
+        step = box / n_voxels
+        return (ctx or context()).mo_grid(self._flat_basis(), self._coefficients, orbitals, (-0.5 * box,) * 3,
+                                          (step,) * 3, (n_voxels,) * 3)
+
+    def points(self, orbital: int, points, ctx: Optional[abi.Context] = None) -> np.ndarray:
+        """MolecularOrbital::evaluate of one orbital at many points [m, 3] (Bohr), float64."""
+        return (ctx or context()).mo_points(self._flat_basis(), self._coefficients, orbital, points)
 
     def __len__(self) -> int:
         return len(self.molecular_orbitals)
@@ -86,6 +109,11 @@ class HartreeFockResult:
     history: list
     scf_wall_ms: float
     converged: bool
+    # options the SCF ran with (integral policy, screen, Schwarz threshold): a nuclear gradient of
+    # this result differentiates the same energy
+    policy: int = abi.POLICY_ACCURATE
+    screen: int = abi.SCREEN_SCHWARZ
+    schwarz_threshold: float = 1e-12
 
 
 def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_charge: int = 0,
@@ -122,10 +150,11 @@ def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_char
         orbital_energies=r["orbital_energies"], density_matrix=r["density"],
         coefficient_matrix=r["coefficients"], electronic_energy=r["electronic_energy"],
         total_energy=r["total_energy"], iterations=r["iterations"], n_electrons=r["n_electrons"],
-        n_basis=r["n_basis"], history=r["history"], scf_wall_ms=r["scf_wall_ms"], converged=(st == abi.FOCK_OK))
+        n_basis=r["n_basis"], history=r["history"], scf_wall_ms=r["scf_wall_ms"], converged=(st == abi.FOCK_OK),
+        policy=policy, screen=screen, schwarz_threshold=schwarz_threshold)
 
 
-def _result(atoms, r, st) -> HartreeFockResult:
+def _result(atoms, r, st, policy, screen, schwarz_threshold) -> HartreeFockResult:
     basis = [cg for a in atoms for cg in a.basis]
     return HartreeFockResult(
         orbitals=MolecularOrbitals(basis, r["coefficients"]),
@@ -133,7 +162,7 @@ def _result(atoms, r, st) -> HartreeFockResult:
         coefficient_matrix=r["coefficients"], electronic_energy=r["electronic_energy"],
         total_energy=r["total_energy"], iterations=r["iterations"], n_electrons=r["n_electrons"],
         n_basis=r["n_basis"], history=r.get("history", []), scf_wall_ms=r["scf_wall_ms"],
-        converged=(st == abi.FOCK_OK))
+        converged=(st == abi.FOCK_OK), policy=policy, screen=screen, schwarz_threshold=schwarz_threshold)
 
 
 def try_scf_batch(molecules: Sequence[Iterable[Atom]], max_iters: int, epsilon: float, molecule_charge: int = 0,
@@ -148,12 +177,23 @@ def try_scf_batch(molecules: Sequence[Iterable[Atom]], max_iters: int, epsilon: 
                             integral_policy=policy, screen=screen, schwarz_threshold=schwarz_threshold, **schedule)
     ctx = ctx or context()
     _, statuses, results = ctx.scf_batch_run(flat, opts)
-    return [None if st == abi.FOCK_DIVERGED else _result(a, r, st) for a, r, st in zip(mols, results, statuses)]
+    return [None if st == abi.FOCK_DIVERGED else _result(a, r, st, policy, screen, schwarz_threshold)
+            for a, r, st in zip(mols, results, statuses)]
 
 
-def nuclear_gradient(atoms: Iterable[Atom], result: HartreeFockResult, screen: int = abi.SCREEN_SCHWARZ,
-                     schwarz_threshold: float = 1e-12, ctx: Optional[abi.Context] = None) -> np.ndarray:
-    """dE/dR in Eh/Bohr, shape (n_atoms, 3), at an accurate-policy try_scf result."""
+def nuclear_gradient(atoms: Iterable[Atom], result: HartreeFockResult, screen: Optional[int] = None,
+                     schwarz_threshold: Optional[float] = None, ctx: Optional[abi.Context] = None) -> np.ndarray:
+    """dE/dR in Eh/Bohr, shape (n_atoms, 3), at an accurate-policy try_scf result. The screen and
+    Schwarz threshold default to the ones the SCF ran with, so the gradient is that of the
+    returned energy. Raises ValueError for a result of the reference-compatible integrals or of
+    the reference's 1e-6 pair screen: neither energy is differentiable (the Boys table is
+    piecewise linear; the pair screen zeroes single integrals)."""
+    if result.policy != abi.POLICY_ACCURATE:
+        raise ValueError("nuclear gradients need an SCF with the accurate integral policy")
+    screen = result.screen if screen is None else screen
+    schwarz_threshold = result.schwarz_threshold if schwarz_threshold is None else schwarz_threshold
+    if screen == abi.SCREEN_REF_BF_PAIR:
+        raise ValueError("nuclear gradients need the Schwarz screen or none, not the 1e-6 pair screen")
     fb, fn = flatten(list(atoms))
     opts = abi.default_opts(integral_policy=abi.POLICY_ACCURATE, screen=screen, schwarz_threshold=schwarz_threshold)
     r = {"coefficients": result.coefficient_matrix, "density": result.density_matrix,

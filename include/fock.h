@@ -29,8 +29,13 @@ typedef enum fock_status {
      * Some(result) in this case (scf/src/lib.rs:224). */
     FOCK_NOT_CONVERGED = 1,
     /* max|FDS - SDF| became NaN/inf/subnormal; the reference returns None
-     * (scf/src/lib.rs:248-249). Outputs are not meaningful. */
+     * (scf/src/lib.rs:248-249). Also returned, before any eigensolve, when the overlap matrix is
+     * singular (smallest eigenvalue <= n * DBL_EPSILON * largest, e.g. coincident atoms), where
+     * the reference's 1/sqrt of (U^T S U)_ii (scf/src/lib.rs:139-146) ends in None. Only
+     * n_electrons and n_basis of the outputs are meaningful. */
     FOCK_DIVERGED = 2,
+    /* Bad arguments, including non-finite centres, exponents, coefficients, nuclear positions or
+     * charges (rejected before anything is launched). */
     FOCK_INVALID_ARG = 3,
     /* Boys order > 10 under the compatibility policy; the reference panics
      * (boys/src/lib.rs:59-62). */
@@ -190,7 +195,8 @@ fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fock_basis_de
  * 1/2 sum (P P - exchange) d(ab|cd) + dE_nuc, W = 2 sum_occ eps_i C_i C_i^T, with the energy of
  * scf/src/lib.rs:223 and the nuclear repulsion of :102-116. Accurate integral policy only (the
  * reference Boys table is not differentiable); opts->screen / schwarz_threshold select the
- * quartets as in the SCF; opts->precision = FOCK_PRECISION_MIXED evaluates the derivative
+ * quartets as in the SCF: FOCK_SCREEN_SCHWARZ or FOCK_SCREEN_NONE (FOCK_SCREEN_REF_BF_PAIR zeroes
+ * single integrals of kept quartets, which leaves the energy non-differentiable: FOCK_INVALID_ARG); opts->precision = FOCK_PRECISION_MIXED evaluates the derivative
  * integrals of quartets with Schwarz bound below 1e-4 in FP32 (FP64 accumulation). Every
  * basis-function centre must coincide with a nucleus. The context's prepared J/K state is the
  * one for this basis and options afterwards: kept when the last preparation (fock_jk_prepare or
@@ -264,9 +270,10 @@ fock_status fock_jk_set_timing(fock_ctx *ctx, int32_t enable);
 fock_status fock_jk_get_timings(fock_ctx *ctx, double ms[6]);
 
 /* Statistics of the prepared J/K build, per quartet class (ss|ss), (ps|ss), (ps|ps), (pp|ss),
- * (pp|ps), (pp|pp). Counts are of work that survives the screen (primitive quartets: the
- * accurate policy's primitive screen at the threshold of the prepared precision's full builds,
- * 1e-15 FP64, 1e-11 for the FP32 kernels of the mixed form). */
+ * (pp|ps), (pp|pp). Counts are of the work one full build of the prepared precision evaluates:
+ * shell quartets of the task lists (the FP32 kernels of the mixed form also drop shell quartets
+ * with Q_ab Q_cd < 1e-11), their integrals, and the primitive quartets the accurate policy's
+ * primitive screen keeps (1e-15 FP64, 1e-11 for the FP32 kernels of the mixed form). */
 typedef struct fock_jk_stats {
     int32_t n_basis, n_shells, n_shell_pairs;
     int64_t shell_quartets[6];
@@ -276,8 +283,44 @@ typedef struct fock_jk_stats {
     double algorithmic_flop[6];  /* canonical McMurchie-Davidson count: f_prim(class) per primitive
                                   * quartet + 12 per contracted integral (J: 2 FMA, K: 4 FMA) */
     int64_t near_threshold;      /* REF_BF_PAIR: |estimate - 1e-6| < 1e-12 pairs met */
+    int64_t preparations;        /* task-list preparations this context has made so far (a gradient
+                                  * that reuses the SCF's lists leaves it unchanged) */
+    uint64_t list_checksum;      /* FNV-1a of the prepared pair records, primitive-pair data, task
+                                  * and block lists: equal for equal inputs, run to run */
 } fock_jk_stats;
 fock_status fock_jk_get_stats(fock_ctx *ctx, fock_jk_stats *stats);
+
+/* --------------------------------------------------------- orbitals at points */
+
+/* A regular grid: voxel (x, y, z), 0 <= x < dims[0] etc., is point origin + (x step[0], y step[1],
+ * z step[2]) (Bohr) and number x + dims[0] (y + dims[1] z). The volumetric renderer's grid
+ * (volumetric/src/main.rs:100-125: N_VOXELS = 200, BOX_SIZE = 20) is dims = 200, origin = -10,
+ * step = 0.1 on every axis. */
+typedef struct fock_grid_desc {
+    double origin[3];
+    double step[3];
+    int32_t dims[3];
+} fock_grid_desc;
+
+/* Molecular orbitals on a grid: out[o * nvox + v] = (float) psi_k(r_v), k = orbitals[o], nvox =
+ * dims[0] dims[1] dims[2]. psi_k is MolecularOrbital::evaluate (scf/src/molecular_orbitals.rs:37-43)
+ * over the functions f with |C_fk| > 0.05 (MolecularOrbitals::new, :69-90), phi_f as
+ * ContractedGaussian::evaluate (basis/src/contracted_gaussian.rs:320-331), evaluated in FP64 and
+ * rounded to float as the renderer's texture (volumetric/src/main.rs:115-117). coefficients:
+ * column-major n x n host array (fock_scf_out's coefficient matrix); out: host array of
+ * n_orbitals * nvox floats. Replaces the rayon loop of create_wave_texture_data
+ * (volumetric/src/main.rs:100-125). */
+fock_status fock_mo_grid(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
+                         const int32_t *orbitals, int32_t n_orbitals, const fock_grid_desc *grid, float *out);
+/* Same into a device array d_out (HBM); the orbital records are uploaded synchronously, the
+ * evaluation is enqueued on the context's stream without a host synchronisation. */
+fock_status fock_mo_grid_device(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
+                                const int32_t *orbitals, int32_t n_orbitals, const fock_grid_desc *grid,
+                                float *d_out);
+/* psi_k at n_points arbitrary points (points[3i..3i+2], Bohr) in FP64: out[i] (host arrays);
+ * the pointcloud renderer's MolecularOrbital::evaluate calls (pointcloud/src/main.rs:106-121). */
+fock_status fock_mo_points(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients, int32_t orbital,
+                           int64_t n_points, const double *points, double *out);
 
 /* ------------------------------------------------------------------ autotuner */
 

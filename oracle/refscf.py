@@ -155,7 +155,8 @@ def density(C: np.ndarray, n_electrons: int) -> np.ndarray:
 
 def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, accurate: bool = False,
             screen: bool = True, integrals=None, P0=None, diis_start: int = 5, diis_window: int = 16,
-            level_shift: float = 0.0, level_shift_until: float = 0.0) -> Optional[ScfResult]:
+            level_shift: float = 0.0, level_shift_until: float = 0.0,
+            exit_plain: Optional[bool] = None) -> Optional[ScfResult]:
     """scf/src/lib.rs:73-258 on the flat basis `fb` and nuclei `fn`.
 
     accurate=False: the reference's own integrals (refcpu.c, LUT Boys, x-branch as written,
@@ -163,7 +164,13 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
     `integrals` may pass precomputed (S, T, V, packed) to skip their evaluation; `P0` replaces
     the GWH guess by a given density. diis_start / diis_window / level_shift(_until) generalise
     the reference's constants (5, 16, no shift) exactly as the C ABI options do.
+    exit_plain (default: `accurate`): on the iteration whose error is below epsilon, diagonalise
+    the newest Fock matrix instead of the DIIS extrapolation -- the rule the GPU driver applies
+    under its accurate policy (an extrapolation from parallel errors is a singular system whose
+    solution follows rounding); the reference's own rule is exit_plain=False.
     """
+    if exit_plain is None:
+        exit_plain = accurate
     n = fb.n
     n_electrons = int(round(float(np.sum(fn.charges)))) - molecule_charge
     e_nuc = nuclear_repulsion(fn.positions, fn.charges)
@@ -199,7 +206,7 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
         if len(errs) > diis_window:
             errs.popleft()
             focks.popleft()
-        if len(focks) < diis_start:
+        if len(focks) < diis_start or (exit_plain and err < epsilon):
             Fx = focks[-1]
         else:
             Fx = diis(errs, focks)

@@ -2,7 +2,8 @@
 
     python tests/golden/make_golden.py small     # H2, H2O (W7), H2O/6-31G: seconds
     python tests/golden/make_golden.py o108      # n-octane/6-31G: ~1 minute
-    python tests/golden/make_golden.py e264      # n-eicosane/6-31G accurate energy: ~5 minutes
+    python tests/golden/make_golden.py e264      # n-eicosane/6-31G accurate SCF, P, J, K: ~5 minutes
+    python tests/golden/make_golden.py forms     # water clusters and gradients: ~5 minutes
 
 Fixtures are plain .npz arrays (no pickles). The oracle is the checker: refcpu.c
 (the reference's arithmetic) and oracle64.c (accurate, unscreened), with the SCF
@@ -117,9 +118,53 @@ def e264():
     out["e264_acc_iters"] = np.array([r.iterations])
     out["e264_acc_errors"] = np.array(r.errors)
     print("e264 accurate E=%.12f iters=%d (%.1fs)" % (r.total_energy, r.iterations, time.time() - t), flush=True)
+    # the converged density and the unscreened J, K it gives (upper triangles, row-major order of
+    # numpy.triu_indices): the reference point of the precision form's max |dF_ij|
+    n = fb.n
+    P = np.ascontiguousarray(r.density_matrix)
+    J = np.zeros((n, n)); K = np.zeros((n, n))
+    from oracle import dptr, lib
+    lib().o64_jk_packed(n, dptr(packed), dptr(P), dptr(J), dptr(K))
+    iu = np.triu_indices(n)
+    out["e264_acc_P_triu"] = P[iu]
+    out["e264_acc_J_triu"] = J[iu]
+    out["e264_acc_K_triu"] = K[iu]
     np.savez_compressed(os.path.join(HERE, "e264.npz"), **out)
+
+
+ROBUST = dict(diis_start=2, diis_window=10, level_shift=0.5, level_shift_until=0.05)
+CLUSTER_SEED = 20261203
+
+
+def forms():
+    """Oracle references of the batching and differentiation forms: energies of seeded (H2O)_k,
+    k = 2, 3, 4 (6-31G); central finite-difference gradients (1e-4 Bohr) of the accurate,
+    unscreened energy for the seeded (H2O)4 of the gradient tests and for HF/6-311G (the largest
+    exponents of the shipped s/p sets: F 1.1e4)."""
+    out = {}
+    rng = np.random.default_rng(CLUSTER_SEED)
+    for k in (2, 3, 4):
+        t = time.time()
+        g = m.water_cluster_geometry(k, rng)
+        fb, fn = m.flatten(m.build(g, "6-31G"))
+        r = refscf.try_scf(fb, fn, 300, 1e-10, 0, accurate=True, **ROBUST)
+        out["w%d_sym" % k] = np.array([a[0] for a in g])
+        out["w%d_xyz" % k] = np.array([a[1:] for a in g], dtype=np.float64)
+        out["w%d_energy" % k] = np.array([r.total_energy])
+        out["w%d_eps" % k] = r.orbital_energies
+        print("(H2O)%d E=%.12f iters=%d (%.1fs)" % (k, r.total_energy, r.iterations, time.time() - t), flush=True)
+    cases = {"grad_w4": (m.water_cluster_geometry(4, np.random.default_rng(4)), "6-31G"),
+             "grad_hf": ([("F", 0.0, 0.0, 0.0), ("H", 0.0, 0.0, 0.917)], "6-311G")}
+    for key, (g, basis) in cases.items():
+        t = time.time()
+        fb, fn = m.flatten(m.build(g, basis))
+        out[key + "_sym"] = np.array([a[0] for a in g])
+        out[key + "_xyz"] = np.array([a[1:] for a in g], dtype=np.float64)
+        out[key + "_fd"] = refscf.fd_gradient(fb, fn, **ROBUST)
+        print("%s fd gradient (%.1fs)" % (key, time.time() - t), flush=True)
+    np.savez_compressed(os.path.join(HERE, "forms.npz"), **out)
 
 
 if __name__ == "__main__":
     for stage in sys.argv[1:] or ["small"]:
-        {"small": small, "o108": o108, "e264": e264}[stage]()
+        {"small": small, "o108": o108, "e264": e264, "forms": forms}[stage]()

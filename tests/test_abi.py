@@ -45,7 +45,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(abi.NucleiDesc) == 24
     assert ctypes.sizeof(abi.ScfOpts) == 96
     assert ctypes.sizeof(abi.ScfIter) == 32
-    assert ctypes.sizeof(abi.JKStats) == 12 + 4 + 6 * 8 * 5 + 8
+    assert ctypes.sizeof(abi.JKStats) == 12 + 4 + 6 * 8 * 5 + 8 + 8 + 8
     assert ctypes.sizeof(abi.TuneReport) == 6 * 4 + 6 * 5 * 8 + 4 + 4 + 2 * 8 + 8
 
 

@@ -13,8 +13,10 @@ ROBUST = dict(diis_start=2, diis_window=10, level_shift=0.5, level_shift_until=0
 
 
 def opts(**kw):
-    return abi.default_opts(max_iters=200, epsilon=1e-9, integral_policy=abi.POLICY_ACCURATE,
-                            screen=abi.SCREEN_SCHWARZ, schwarz_threshold=1e-12, **{**ROBUST, **kw})
+    base = dict(max_iters=200, epsilon=1e-9, integral_policy=abi.POLICY_ACCURATE, screen=abi.SCREEN_SCHWARZ,
+                schwarz_threshold=1e-12, **ROBUST)
+    base.update(kw)
+    return abi.default_opts(**base)
 
 
 def clusters(count, seed):
@@ -61,6 +63,27 @@ def test_batch_above_jacobi_limit_uses_rocsolver(ctx):
     st, statuses, res = ctx.scf_batch_run(mols, o)
     assert st == abi.FOCK_OK, statuses
     check_against_single(ctx, mols, res, o, range(len(mols)))
+
+
+def forms_geometry(d, key):
+    return [(str(s), float(x), float(y), float(z)) for s, (x, y, z) in zip(d[key + "_sym"], d[key + "_xyz"])]
+
+
+def test_batch_matches_oracle_energies(ctx):
+    """Seeded (H2O)_2, (H2O)_3, (H2O)_4 in one batch against the accurate, unscreened oracle SCF
+    energies (tests/golden/forms.npz): |dE| <= 1e-8 Eh, orbital energies within 1e-6 Eh."""
+    import os
+    from conftest import REPO
+    d = np.load(os.path.join(REPO, "tests", "golden", "forms.npz"))
+    mols = [m.flatten(m.build(forms_geometry(d, "w%d" % k), "6-31G")) for k in (2, 3, 4)]
+    o = opts(epsilon=1e-10)
+    st, statuses, res = ctx.scf_batch_run(mols, o)
+    assert st == abi.FOCK_OK, statuses
+    for k, r in zip((2, 3, 4), res):
+        dE = abs(r["total_energy"] - d["w%d_energy" % k][0])
+        print("(H2O)%d batch |dE| vs oracle %.2e" % (k, dE))
+        assert dE <= 1e-8
+        assert np.abs(r["orbital_energies"] - d["w%d_eps" % k]).max() < 1e-6
 
 
 def test_batch_rejects_mixed_precision(ctx):

@@ -58,6 +58,25 @@ def test_gradient_water_vs_oracle_fd(ctx, basis):
     assert np.abs(g.sum(axis=0)).max() < 1e-10
 
 
+@pytest.mark.parametrize("key,basis", [("grad_w4", "6-31G"), ("grad_hf", "6-311G")])
+def test_gradient_vs_oracle_fd_fixture(ctx, key, basis):
+    """The seeded (H2O)4 and HF/6-311G (F exponents to 1.1e4: the gradient's primitive cut scales
+    with the largest exponent) against central finite differences of the oracle's accurate,
+    unscreened energy (tests/golden/forms.npz, step 1e-4 Bohr)."""
+    import os
+    from conftest import REPO
+    d = np.load(os.path.join(REPO, "tests", "golden", "forms.npz"))
+    geom = [(str(s), float(x), float(y), float(z)) for s, (x, y, z) in zip(d[key + "_sym"], d[key + "_xyz"])]
+    fb, fn = m.flatten(m.build(geom, basis))
+    o = opts(screen=abi.SCREEN_SCHWARZ, tau=1e-13, eps=1e-12, **ROBUST)
+    r = energy(ctx, fb, fn, o)
+    g = ctx.rhf_gradient(fb, fn, o, r)
+    err = np.abs(g - d[key + "_fd"]).max()
+    print("%s max |analytic - oracle fd| = %.2e" % (key, err))
+    assert err < TOL
+    assert np.abs(g.sum(axis=0)).max() < 1e-10
+
+
 def test_gradient_water_tetramer_vs_gpu_fd(ctx):
     rng = np.random.default_rng(4)
     fb, fn = m.flatten(m.build(m.water_cluster_geometry(4, rng), "6-31G"))
@@ -102,6 +121,18 @@ def test_gradient_reuses_or_rebuilds_prepared_lists(ctx):
     assert np.abs(gb - ga).max() > 1e-3
     assert np.abs(ga2 - ga).max() < 1e-12
     assert np.abs(ga3 - ga).max() < 1e-9
+    # the preparation count: a gradient after the SCF of the same basis and options reuses its
+    # lists, one after another basis or threshold prepares again
+    p0 = ctx.jk_stats()["preparations"]
+    energy(ctx, fa, na, o)                       # SCF of A: one preparation
+    p1 = ctx.jk_stats()["preparations"]
+    ctx.rhf_gradient(fa, na, o, ra)              # reuses A's lists
+    p2 = ctx.jk_stats()["preparations"]
+    ctx.rhf_gradient(fbb, nb, o, rb)             # B: prepares
+    p3 = ctx.jk_stats()["preparations"]
+    ctx.rhf_gradient(fbb, nb, opts(screen=abi.SCREEN_SCHWARZ, tau=1e-14, eps=1e-12), rb)  # new threshold
+    p4 = ctx.jk_stats()["preparations"]
+    assert (p1 - p0, p2 - p1, p3 - p2, p4 - p3) == (1, 0, 1, 1)
 
 
 def test_gradient_rejects_reference_policy(ctx):

@@ -147,15 +147,35 @@ def test_jk_o108(ctx, o108, policy, screen, key, tol):
     assert st["n_basis"] == 108 and st["n_shells"] == 76
     if screen == abi.SCREEN_NONE:
         assert sum(st["shell_quartets"]) == 4282201  # unique shell quartets (s and p of SP counted apart)
+    if screen == abi.SCREEN_REF_BF_PAIR:
+        # no basis-function pair product within 1e-12 of the reference's 1e-6 threshold: the keep /
+        # zero decisions cannot flip on last-bit differences of the diagonal integrals
+        assert st["near_threshold"] == 0
 
 
-def test_screen_lists_are_bit_exact_between_runs(ctx):
+@pytest.mark.parametrize("policy,screen", [(abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ),
+                                           (abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR)])
+def test_screen_lists_are_bit_exact_between_runs(ctx, policy, screen):
+    """The checksum covers the sorted pair records, the primitive-pair data, the task and the
+    block lists; preparing the same basis again (after another molecule) gives the same lists,
+    a different threshold gives different ones."""
     fb, _ = m.flatten(m.octane_631g())
-    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12)
+    other, _ = m.flatten(m.build(m.water_geometry(), "6-31G"))
+    ctx.set_tiles(abi.TILES_FP64, [8] * 6)
+    ctx.jk_prepare(fb, policy, screen, 1e-12)
     a = ctx.jk_stats()
-    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12)
+    ctx.jk_prepare(other, policy, screen, 1e-12)
+    c = ctx.jk_stats()
+    ctx.jk_prepare(fb, policy, screen, 1e-12)
     b = ctx.jk_stats()
-    assert a == b
+    assert a["list_checksum"] == b["list_checksum"] != c["list_checksum"]
+    assert {k: v for k, v in a.items() if k != "preparations"} == {k: v for k, v in b.items() if k != "preparations"}
+    if screen == abi.SCREEN_SCHWARZ:
+        ctx.jk_prepare(fb, policy, screen, 1e-10)
+        assert ctx.jk_stats()["list_checksum"] != a["list_checksum"]
+    else:
+        # the reference's 1e-6 pair screen: no basis-function pair product within 1e-12 of it here
+        assert a["near_threshold"] == 0
 
 
 # ------------------------------------------------------------------ SCF

@@ -328,3 +328,119 @@ def test_fd_gradient_h2_matches_energy_curve():
         e.append(refscf.try_scf(fb2, fn2, 200, 1e-13, 0, accurate=True).total_energy)
     dEdR = (e[0] - e[1]) / 2e-3
     assert abs(g[1, 2] - dEdR) < 1e-6 and abs(g[0, 2] + dEdR) < 1e-6
+
+
+# ------------------------------------------------------------------ pins of the J/K contraction,
+# the reference screen, DIIS and the orbital evaluator
+
+def _random_symmetric(n, seed):
+    A = np.random.default_rng(seed).standard_normal((n, n))
+    return np.ascontiguousarray(A + A.T)
+
+
+@pytest.mark.parametrize("which", ["w7", "w13"])
+def test_o64_jk_packed_equals_dense_einsum(oracle_lib, which):
+    """o64_jk_packed (which wrote every J/K fixture) against numpy.einsum over the unpacked dense
+    tensor: J_ij = sum_kl (ij|kl) P_kl, K_ij = sum_kl (ik|jl) P_kl, at a random symmetric P."""
+    import oracle
+    from oracle import refscf
+    atoms = m.h2o_sto3g() if which == "w7" else m.build(m.water_geometry(), "6-31G")
+    fb, _ = m.flatten(atoms)
+    n = fb.n
+    packed, _ = refscf.eri_packed(fb, True)
+    D = refscf.dense_from_packed(n, packed)
+    P = _random_symmetric(n, 5)
+    J = np.zeros((n, n)); K = np.zeros((n, n))
+    oracle_lib.o64_jk_packed(n, oracle.dptr(packed), oracle.dptr(P), oracle.dptr(J), oracle.dptr(K))
+    Jw = np.einsum("ijkl,kl->ij", D, P)
+    Kw = np.einsum("ikjl,kl->ij", D, P)
+    scale = np.abs(D).max() * np.abs(P).sum()
+    assert np.abs(J - Jw).max() < 1e-14 * scale
+    assert np.abs(K - Kw).max() < 1e-14 * scale
+    assert np.abs(J - J.T).max() == 0.0 and np.abs(K - K.T).max() == 0.0
+
+
+@pytest.mark.parametrize("which", ["w13", "dimer6"])
+def test_reference_screen_decision(oracle_lib, which):
+    """ref_electron_tensor's 1e-6 rule (basis/src/electron_tensor.rs:89-101, 134-155) on
+    H2O/6-31G (where nothing falls below it) and on two waters 6 A apart (where it drops
+    quartets): every diagonal quartet (xy|xy) is kept; any other slot is zero exactly when
+    sqrt((xy|xy)(zw|zw)) <= 1e-6 and otherwise equals the unscreened value bit for bit."""
+    from oracle import refscf
+    geom = m.water_geometry()
+    if which == "dimer6":
+        geom = geom + [(s, x + 6.0, y, z) for s, x, y, z in geom]
+    fb, _ = m.flatten(m.build(geom, "6-31G"))
+    n = fb.n
+    scr, kept_count = refscf.eri_packed(fb, False, screen=True)
+    full, all_count = refscf.eri_packed(fb, False, screen=False)
+    Ds = refscf.dense_from_packed(n, scr)
+    Df = refscf.dense_from_packed(n, full)
+    d = np.einsum("ijij->ij", Df)
+    est = np.sqrt(d[:, :, None, None] * d[None, None, :, :])
+    i, j, k, l = np.indices((n,) * 4)
+    diag = ((i == k) & (j == l)) | ((i == l) & (j == k))
+    keep = diag | (est > 1e-6)
+    assert np.array_equal(Ds[keep], Df[keep])
+    assert np.all(Ds[~keep] == 0.0)
+    if which == "dimer6":
+        assert 0 < np.count_nonzero(~keep)        # the screen does drop quartets here
+        assert kept_count < all_count
+
+
+def test_refscf_diis_is_the_constrained_least_squares_minimum():
+    """refscf.diis: the coefficients sum to one (focks all equal to one matrix give that matrix),
+    and with focks = errors the extrapolated error is the minimum of |sum c_i e_i| over
+    sum c_i = 1, found independently with numpy.linalg.lstsq on the null space of the constraint."""
+    from oracle import refscf
+    rng = np.random.default_rng(12)
+    errs = [rng.standard_normal((6, 6)) * 10.0 ** (-0.5 * k) for k in range(5)]
+    ones = [np.full((6, 6), 3.0) for _ in errs]
+    assert np.abs(refscf.diis(errs, ones) - 3.0).max() < 1e-12
+    got = refscf.diis(errs, errs)
+    E = np.stack([e.ravel() for e in errs], axis=1)          # columns e_i
+    m_ = E.shape[1]
+    c0 = np.full(m_, 1.0 / m_)
+    N = np.linalg.svd(np.ones((1, m_)))[2][1:].T            # basis of {t : sum t = 0}
+    t, *_ = np.linalg.lstsq(E @ N, -(E @ c0), rcond=None)
+    want = (E @ (c0 + N @ t)).reshape(6, 6)
+    assert np.abs(got - want).max() < 1e-10 * np.abs(want).max()
+
+
+def test_refmo_basis_values_integrate_to_the_overlap():
+    """refmo.basis_values (ContractedGaussian::evaluate restated) integrated on a grid: the
+    products of the diffuse valence functions of H2O/6-31G (exponents below 1, so a 0.4 Bohr
+    trapezoid grid is exact to ~1e-12 for Gaussians) give the oracle's analytic overlap integrals,
+    including the sign of p functions on one centre against s functions on another."""
+    from oracle import refmo, refscf
+    fb, fn = m.flatten(m.build(m.water_geometry(), "6-31G"))
+    diffuse = [f for f in range(fb.n)
+               if np.all(fb.exponents[fb.prim_offset[f]:fb.prim_offset[f + 1]] < 1.0)]
+    assert any(fb.angular[f].sum() == 1 for f in diffuse) and len(diffuse) >= 5
+    h, L = 0.4, 14.0
+    ax = np.arange(-L, L + h / 2, h)
+    z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    V = refmo.basis_values(fb, pts, diffuse)
+    Snum = (V @ V.T) * h ** 3
+    S, _, _ = refscf.one_electron(fb, fn, True)
+    Sref = S[np.ix_(diffuse, diffuse)]
+    assert np.abs(Snum - Sref).max() < 1e-9
+    assert np.abs(Sref - np.diag(np.diag(Sref))).max() > 0.1   # off-diagonal overlaps are tested
+
+
+def test_refmo_truncation_and_voxel_order():
+    """MolecularOrbitals::new keeps |c| > 0.05 strictly (molecular_orbitals.rs:79); voxel i of the
+    renderer is (i mod N, (i / N) mod N, i / N^2) at (v / N - 0.5) BOX (volumetric/src/main.rs:111-116)."""
+    from oracle import refmo
+    C = np.array([[0.05, -0.0500001], [0.2, 0.0], [-0.05, 0.049], [0.051, -1.0]])
+    idx, c = refmo.truncate(C, 0)
+    assert list(idx) == [1, 3] and list(c) == [0.2, 0.051]
+    idx, c = refmo.truncate(C, 1)
+    assert list(idx) == [0, 3]
+    N, box = 5, 20.0
+    pts = refmo.volumetric_points(N, box)
+    for i in range(N ** 3):
+        x, y, z = i % N, (i // N) % N, i // N ** 2
+        assert np.array_equal(pts[i], np.array([x, y, z], dtype=np.float64) / N * box - 0.5 * box) or \
+            np.abs(pts[i] - (np.array([x, y, z]) / N - 0.5) * box).max() < 1e-14
