@@ -174,6 +174,7 @@ struct fock_ctx {
     std::string err;
     DevBuf lut, tab64, tab32, pd64, pd32;
     DevBuf mo_shells, mo_funs, mo_dbl, mo_off, mo_pts;  // records of the last orbital evaluation
+    void *mo_scratch = nullptr;  // this device's fixed record array (fock_mo_grid_device_noalloc)
     BoysTables bt{};
     Prepared jk;
     // tasks per block of the blocked digestion per quartet class: [FOCK_TILES_FP64] one molecule,
@@ -401,6 +402,7 @@ extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
         ctx->max_lds = (size_t)lds;
     }
     HIPCHK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+    HIPCHK(mo_scratch_address(device, &ctx->mo_scratch));
     ctx->stream = ctx->own;
     if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
         return ctx->fail(FOCK_HIP_ERROR, "rocblas_create_handle failed");
@@ -1670,6 +1672,55 @@ extern "C" fock_status fock_mo_grid_device(fock_ctx *ctx, const fock_basis_desc 
     int64_t nvox = 0;
     STCHK(check_grid(ctx, grid, nvox));
     return mo_eval(ctx, basis, coefficients, orbitals, n_orbitals, grid, nullptr, nvox, d_out, true);
+}
+
+// The allocation-free variant of fock_mo_grid_device: the orbital records are packed into the
+// device's fixed array (MO_SCRATCH_BYTES) instead of context buffers; inputs over that size are
+// rejected. Calls on one device share the array: calls ordered on one stream are safe, calls that
+// overlap on one device (other streams or contexts) race.
+extern "C" fock_status fock_mo_grid_device_noalloc(fock_ctx *ctx, const fock_basis_desc *basis,
+                                                   const double *coefficients, const int32_t *orbitals,
+                                                   int32_t n_orbitals, const fock_grid_desc *grid, float *d_out) {
+    if (!ctx) return FOCK_INVALID_ARG;
+    int64_t nvox = 0;
+    STCHK(check_grid(ctx, grid, nvox));
+    if (!basis || !coefficients || !orbitals || n_orbitals <= 0 || n_orbitals > 65535 || !d_out)
+        return ctx->fail(FOCK_INVALID_ARG, "bad orbital evaluation arguments");
+    if (!ctx->mo_scratch) return ctx->fail(FOCK_HIP_ERROR, "no record array on this device");
+    MoHost h;
+    STCHK(build_mo_records(ctx, basis, coefficients, orbitals, n_orbitals, h));
+    // packed layout: shell offsets | shells | functions | doubles, each part 256-byte aligned
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_off = h.shell_off.size() * sizeof(int), b_sh = h.shells.size() * sizeof(MoShellRec),
+                 b_fn = h.funs.size() * sizeof(MoFunRec), b_db = h.dbl.size() * sizeof(double);
+    const size_t o_sh = up(b_off), o_fn = o_sh + up(b_sh), o_db = o_fn + up(b_fn), total = o_db + b_db;
+    if (total > MO_SCRATCH_BYTES)
+        return ctx->fail(FOCK_INVALID_ARG, "orbital records need " + std::to_string(total) + " bytes, more than the " +
+                                               std::to_string(MO_SCRATCH_BYTES) +
+                                               " of the allocation-free path: use fock_mo_grid_device");
+    HIPCHK(hipSetDevice(ctx->device));
+    char *base = static_cast<char *>(ctx->mo_scratch);
+    HIPCHK(hipMemcpyAsync(base, h.shell_off.data(), b_off, hipMemcpyHostToDevice, S(ctx)));
+    if (b_sh) HIPCHK(hipMemcpyAsync(base + o_sh, h.shells.data(), b_sh, hipMemcpyHostToDevice, S(ctx)));
+    if (b_fn) HIPCHK(hipMemcpyAsync(base + o_fn, h.funs.data(), b_fn, hipMemcpyHostToDevice, S(ctx)));
+    if (b_db) HIPCHK(hipMemcpyAsync(base + o_db, h.dbl.data(), b_db, hipMemcpyHostToDevice, S(ctx)));
+    MoArgs a{};
+    a.shell_off = reinterpret_cast<const int *>(base);
+    a.shells = reinterpret_cast<const MoShellRec *>(base + o_sh);
+    a.funs = reinterpret_cast<const MoFunRec *>(base + o_fn);
+    a.dbl = reinterpret_cast<const double *>(base + o_db);
+    a.norb = n_orbitals;
+    a.npoints = nvox;
+    a.out = d_out;
+    a.out_stride = nvox;
+    for (int x = 0; x < 3; ++x) {
+        a.origin[x] = grid->origin[x];
+        a.step[x] = grid->step[x];
+        a.dims[x] = grid->dims[x];
+    }
+    HIPCHK(hipStreamSynchronize(S(ctx)));  // the host records go out of scope
+    HIPCHK(launch_mo_eval(a, true, true, S(ctx)));
+    return FOCK_OK;
 }
 
 extern "C" fock_status fock_mo_points(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,

@@ -286,6 +286,11 @@ struct MoArgs {
     int64_t out_stride;
 };
 hipError_t launch_mo_eval(const MoArgs &a, bool grid, bool f32, hipStream_t s);
+// the allocation-free orbital evaluation keeps its records in a fixed device array of this size
+// (one per device); its address on `device` (which must be current), looked up once per device
+constexpr size_t MO_SCRATCH_BYTES = (size_t)4 << 20;
+constexpr int MO_SCRATCH_MAX_DEVICES = 64;
+hipError_t mo_scratch_address(int device, void **out);
 
 hipError_t launch_eri_diag(int pair_class, const DiagArgs &a, int policy, hipStream_t s);
 hipError_t launch_eri_blocks(int cls, const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s);

@@ -11,6 +11,8 @@
 // on the host. Primitives with alpha |d|^2 > MO_EXP_CUT are skipped (exp below 1e-52: under FP32's
 // smallest subnormal, and 1e-50 of any orbital value that matters in FP64). The orbital's records
 // are uniform over the workgroup (orbital = blockIdx.y), so they are read as broadcast loads.
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace fock {
@@ -18,6 +20,13 @@ namespace fock {
 namespace {
 
 constexpr double MO_EXP_CUT = 120.0;
+
+// Records of the allocation-free entry point (fock_mo_grid_device_noalloc): one fixed array per
+// device, held until the process ends. Host code never passes `scratch` itself to a kernel: its
+// per-device address comes from hipGetSymbolAddress (mo_scratch_address).
+static __device__ __align__(256) unsigned char scratch[MO_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MO_SCRATCH_MAX_DEVICES] = {};
 
 __device__ inline double ipow(double x, int e) {
     double r = 1.0;
@@ -76,6 +85,23 @@ __global__ __launch_bounds__(256) void mo_eval_kernel(MoArgs a) {
 }
 
 }  // namespace
+
+hipError_t mo_scratch_address(int device, void **out) {
+    if (device < 0 || device >= MO_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 hipError_t launch_mo_eval(const MoArgs &a, bool grid, bool f32, hipStream_t s) {
     if (a.npoints <= 0 || a.norb <= 0) return hipSuccess;

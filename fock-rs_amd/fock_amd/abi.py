@@ -48,7 +48,7 @@ EXPORTED = [
     "fock_jk_prepare", "fock_jk_build", "fock_jk_build_device", "fock_jk_get_stats", "fock_gemm",
     "fock_sorted_eigs", "fock_diis", "fock_jk_set_timing", "fock_jk_get_timings",
     "fock_jk_autotune", "fock_tiles_get", "fock_tiles_set", "fock_sorted_eigs_batch",
-    "fock_mo_grid", "fock_mo_grid_device", "fock_mo_points",
+    "fock_mo_grid", "fock_mo_grid_device", "fock_mo_points", "fock_mo_grid_device_noalloc",
 ]
 
 
@@ -153,6 +153,7 @@ def load(path: Optional[str] = None):
         "fock_mo_grid": (ctypes.c_int, [vp, bp, _dp, _ip, i32, ctypes.POINTER(GridDesc), ctypes.POINTER(ctypes.c_float)]),
         "fock_mo_grid_device": (ctypes.c_int, [vp, bp, _dp, _ip, i32, ctypes.POINTER(GridDesc), vp]),
         "fock_mo_points": (ctypes.c_int, [vp, bp, _dp, i32, i64, _dp, _dp]),
+        "fock_mo_grid_device_noalloc": (ctypes.c_int, [vp, bp, _dp, _ip, i32, ctypes.POINTER(GridDesc), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -410,6 +411,16 @@ class Context:
         bd = basis_desc(fb)
         self.check(self.lib.fock_mo_grid_device(self.ptr, ctypes.byref(bd), _d(C), _i(orb), len(orb),
                                                 ctypes.byref(g), ctypes.c_void_p(d_out)))
+
+    def mo_grid_device_noalloc(self, fb, coefficients, orbitals, origin, step, dims, d_out: int):
+        """fock_mo_grid_device_noalloc: the same evaluation with no device allocation (records in
+        the device's fixed 4 MiB array)."""
+        orb = np.ascontiguousarray(np.atleast_1d(orbitals), dtype=np.int32)
+        g = GridDesc((ctypes.c_double * 3)(*origin), (ctypes.c_double * 3)(*step), (ctypes.c_int32 * 3)(*dims))
+        C = self._coef_cm(coefficients)
+        bd = basis_desc(fb)
+        self.check(self.lib.fock_mo_grid_device_noalloc(self.ptr, ctypes.byref(bd), _d(C), _i(orb), len(orb),
+                                                        ctypes.byref(g), ctypes.c_void_p(d_out)))
 
     def mo_points(self, fb, coefficients, orbital: int, points) -> np.ndarray:
         """One orbital at points [m, 3] (Bohr), float64."""

@@ -317,6 +317,15 @@ fock_status fock_mo_grid(fock_ctx *ctx, const fock_basis_desc *basis, const doub
 fock_status fock_mo_grid_device(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
                                 const int32_t *orbitals, int32_t n_orbitals, const fock_grid_desc *grid,
                                 float *d_out);
+/* fock_mo_grid_device without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, a renderer's frame loop, a
+ * serving loop, harnesses that forbid hipMalloc). The orbital records go into a fixed array of
+ * 4 MiB per device, held until the process ends (about 100 orbitals of a 264-function basis);
+ * larger inputs are FOCK_INVALID_ARG. Calls on one device share that array: calls ordered on one
+ * stream are safe, calls that overlap on one device (other streams or contexts) race. */
+fock_status fock_mo_grid_device_noalloc(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients,
+                                        const int32_t *orbitals, int32_t n_orbitals, const fock_grid_desc *grid,
+                                        float *d_out);
 /* psi_k at n_points arbitrary points (points[3i..3i+2], Bohr) in FP64: out[i] (host arrays);
  * the pointcloud renderer's MolecularOrbital::evaluate calls (pointcloud/src/main.rs:106-121). */
 fock_status fock_mo_points(fock_ctx *ctx, const fock_basis_desc *basis, const double *coefficients, int32_t orbital,
