@@ -248,6 +248,17 @@ hipError_t sub_f64(int64_t len, const double *a, const double *b, double *out, h
     return hipGetLastError();
 }
 
+__global__ void mix_kernel(int64_t len, double lam, const double *__restrict__ a, const double *__restrict__ b,
+                           double *__restrict__ out) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < len) out[x] = lam * a[x] + (1.0 - lam) * b[x];
+}
+hipError_t mix_f64(int64_t len, double lam, const double *a, const double *b, double *out, hipStream_t s) {
+    if (len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mix_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, lam, a, b, out);
+    return hipGetLastError();
+}
+
 __global__ void fill_kernel(int64_t len, double v, double *__restrict__ out) {
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x < len) out[x] = v;

@@ -470,6 +470,10 @@ extern "C" void fock_scf_opts_default(fock_scf_opts *o) {
     o->level_shift = 0.0;
     o->level_shift_until = 0.0;
     o->autotune = 0;
+    o->ediis = 0;
+    o->ediis_until = 0.1;
+    o->damping = 1.0;  // scf/src/lib.rs:218: F = 1.0
+    o->damping_until = 0.0;
 }
 
 // ====================================================================== Boys
@@ -1805,6 +1809,93 @@ bool diis_solve(int m, const std::vector<double> &Bm, std::vector<double> &c) {
     return true;
 }
 
+// EDIIS coefficients: min_c e.c + 1/2 c^T Q c over the simplex c >= 0, sum c = 1 (m <= 16), by a
+// primal active-set method from the vertex of the lowest e. Each step solves the equality-
+// constrained problem on the free set, [Q_FF 1; 1^T 0] [c_F; lam] = [-e_F; 1], moves towards it
+// until a coefficient reaches zero (that index leaves the free set), and when the free-set optimum
+// is reached frees the bound index with the most negative multiplier e_i + (Q c)_i + lam. False
+// when a free-set system is singular or the iteration does not settle (the caller uses DIIS).
+bool simplex_qp(int m, const std::vector<double> &e, const std::vector<double> &Q, std::vector<double> &c) {
+    c.assign(m, 0.0);
+    int k0 = 0;
+    for (int i = 1; i < m; ++i)
+        if (e[i] < e[k0]) k0 = i;
+    c[k0] = 1.0;
+    std::vector<char> freev(m, 0);
+    freev[k0] = 1;
+    for (int it = 0; it < 200; ++it) {
+        std::vector<int> F;
+        for (int i = 0; i < m; ++i)
+            if (freev[i]) F.push_back(i);
+        const int f = (int)F.size(), N = f + 1;
+        std::vector<double> A((size_t)N * (N + 1), 0.0);
+        for (int a = 0; a < f; ++a) {
+            for (int b = 0; b < f; ++b) A[(size_t)a * (N + 1) + b] = Q[(size_t)F[a] * m + F[b]];
+            A[(size_t)a * (N + 1) + f] = 1.0;
+            A[(size_t)f * (N + 1) + a] = 1.0;
+            A[(size_t)a * (N + 1) + N] = -e[F[a]];
+        }
+        A[(size_t)f * (N + 1) + N] = 1.0;
+        for (int col = 0; col < N; ++col) {  // Gauss-Jordan with partial pivoting
+            int piv = col;
+            for (int r = col + 1; r < N; ++r)
+                if (std::fabs(A[(size_t)r * (N + 1) + col]) > std::fabs(A[(size_t)piv * (N + 1) + col])) piv = r;
+            if (!(std::fabs(A[(size_t)piv * (N + 1) + col]) > 1e-300)) return false;
+            if (piv != col)
+                for (int k = 0; k <= N; ++k) std::swap(A[(size_t)piv * (N + 1) + k], A[(size_t)col * (N + 1) + k]);
+            for (int r = 0; r < N; ++r) {
+                if (r == col) continue;
+                const double g = A[(size_t)r * (N + 1) + col] / A[(size_t)col * (N + 1) + col];
+                if (g == 0.0) continue;
+                for (int k = col; k <= N; ++k) A[(size_t)r * (N + 1) + k] -= g * A[(size_t)col * (N + 1) + k];
+            }
+        }
+        std::vector<double> x(f);
+        for (int a = 0; a < f; ++a) x[a] = A[(size_t)a * (N + 1) + N] / A[(size_t)a * (N + 1) + a];
+        const double lam = A[(size_t)f * (N + 1) + N] / A[(size_t)f * (N + 1) + f];
+        if (!std::isfinite(lam)) return false;
+        double step = 0.0;
+        for (int a = 0; a < f; ++a) {
+            if (!std::isfinite(x[a])) return false;
+            step = std::max(step, std::fabs(x[a] - c[F[a]]));
+        }
+        if (step < 1e-13) {
+            int worst = -1;
+            double mu_min = -1e-12;
+            for (int i = 0; i < m; ++i) {
+                if (freev[i]) continue;
+                double g = e[i] + lam;
+                for (int j = 0; j < m; ++j) g += Q[(size_t)i * m + j] * c[j];
+                if (g < mu_min) {
+                    mu_min = g;
+                    worst = i;
+                }
+            }
+            if (worst < 0) return true;  // every multiplier is non-negative: optimal
+            freev[worst] = 1;
+            continue;
+        }
+        double alpha = 1.0;
+        int block = -1;
+        for (int a = 0; a < f; ++a) {
+            const double p = x[a] - c[F[a]];
+            if (p < 0.0) {
+                const double t = -c[F[a]] / p;
+                if (t < alpha) {
+                    alpha = t;
+                    block = F[a];
+                }
+            }
+        }
+        for (int a = 0; a < f; ++a) c[F[a]] += alpha * (x[a] - c[F[a]]);
+        if (block >= 0) {
+            c[block] = 0.0;
+            freev[block] = 0;
+        }
+    }
+    return false;
+}
+
 }  // namespace
 
 extern "C" fock_status fock_gemm(fock_ctx *ctx, int32_t transA, int32_t transB, int32_t m, int32_t n, int32_t k,
@@ -1951,6 +2042,8 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     if (opts.precision != FOCK_PRECISION_FP64 || opts.incremental)
         return ctx->fail(FOCK_UNSUPPORTED, "the batched SCF runs the FP64 form");
     if (opts.initial_density) return ctx->fail(FOCK_UNSUPPORTED, "no initial density in the batched SCF");
+    if (opts.ediis || (opts.damping > 0.0 && opts.damping < 1.0))
+        return ctx->fail(FOCK_UNSUPPORTED, "EDIIS and damping are options of the single-molecule SCF");
     if (opts.diis_window < 1 || opts.diis_window > 16 || opts.diis_start < 1)
         return ctx->fail(FOCK_INVALID_ARG, "DIIS window must be 1..16 and start >= 1");
     const auto t_start = std::chrono::steady_clock::now();
@@ -2748,6 +2841,10 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     if (opts.diis_window < 1 || opts.diis_window > 16 || opts.diis_start < 1)
         return ctx->fail(FOCK_INVALID_ARG, "DIIS window must be 1..16 and start >= 1");
     if (!(opts.level_shift >= 0.0)) return ctx->fail(FOCK_INVALID_ARG, "negative level shift");
+    if (!std::isfinite(opts.damping) || !std::isfinite(opts.damping_until) || !std::isfinite(opts.ediis_until))
+        return ctx->fail(FOCK_INVALID_ARG, "non-finite damping or EDIIS threshold");
+    const bool damping_on = opts.damping > 0.0 && opts.damping < 1.0;
+    const bool ediis_on = opts.ediis != 0;
     const auto t_start = std::chrono::steady_clock::now();
     const bool debug = std::getenv("FOCK_DEBUG") != nullptr;
     // FOCK_DEBUG: wall time of each phase (synchronises the stream, so only for diagnosis)
@@ -2800,6 +2897,16 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     HIPCHK(dCoef.ensure(16 * sizeof(double)));
     HIPCHK(dSlots.ensure(16 * sizeof(int)));
     HIPCHK(fill_f64(16 * nn, 0.0, dErrs.as<double>(), S(ctx)));
+    // EDIIS: the density of every stored Fock matrix, and per iteration tr(P_new F_j), tr(P_j F_new)
+    // over the 16 ring slots and tr(P_new H); damping: the density before the step
+    DevBuf dPs, dEd, dPold;
+    if (ediis_on) {
+        HIPCHK(dPs.ensure(16 * bytes));
+        HIPCHK(dEd.ensure(40 * sizeof(double)));
+        HIPCHK(fill_f64(16 * nn, 0.0, dPs.as<double>(), S(ctx)));
+        HIPCHK(fill_f64(16 * nn, 0.0, dFocks.as<double>(), S(ctx)));
+    }
+    if (damping_on) HIPCHK(dPold.ensure(bytes));
     double *S_ = dS.as<double>(), *H_ = dH.as<double>(), *X_ = dX.as<double>(), *U_ = dU.as<double>();
     double *T1 = dTmp.as<double>(), *T2 = dTmp2.as<double>(), *P_ = dP.as<double>(), *G_ = dG.as<double>();
     double *F_ = dF.as<double>(), *Fx_ = dFx.as<double>(), *C_ = dC.as<double>();
@@ -2946,6 +3053,8 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     }
     std::deque<int> slots;        // ring slots of the stored (error, Fock) pairs, oldest first
     std::deque<std::vector<double>> dots;  // dots[i][j] = <e_i, e_j> for stored i, j (same order)
+    std::deque<std::vector<double>> pf;    // EDIIS: pf[i][j] = tr(P_i F_j) for stored i, j
+    std::deque<double> ph;                 // EDIIS: tr(P_i H)
     int next_slot = 0;
     hipEvent_t ev0, ev1;
     HIPCHK(hipEventCreate(&ev0));
@@ -3008,11 +3117,23 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         HIPCHK(fill_f64(1, 0.0, scal, S(ctx)));
         HIPCHK(antisym_maxabs(n, T2, e_slot, scal, S(ctx)));
         HIPCHK(copy_f64(nn, F_, f_slot, S(ctx)));
+        if (ediis_on) {
+            double *p_slot = dPs.as<double>() + (int64_t)slot * nn;
+            HIPCHK(copy_f64(nn, P_, p_slot, S(ctx)));
+            HIPCHK(frob_dots(nn, p_slot, dFocks.as<double>(), nn, 16, dEd.as<double>(), S(ctx)));       // tr(P_new F_j)
+            HIPCHK(frob_dots(nn, f_slot, dPs.as<double>(), nn, 16, dEd.as<double>() + 16, S(ctx)));     // tr(P_j F_new)
+            HIPCHK(frob_dots(nn, p_slot, H_, nn, 1, dEd.as<double>() + 32, S(ctx)));                  // tr(P_new H)
+        }
         // window of diis_window, the reference's 16 (:194-200)
         if ((int)slots.size() == opts.diis_window) {
             slots.pop_front();
             dots.pop_front();
             for (auto &row : dots) row.erase(row.begin());
+            if (ediis_on) {
+                pf.pop_front();
+                ph.pop_front();
+                for (auto &row : pf) row.erase(row.begin());
+            }
         }
         slots.push_back(slot);
         const int m = (int)slots.size();
@@ -3021,12 +3142,20 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         {
             // dots against all 16 ring slots, pick the stored ones
             HIPCHK(frob_dots(nn, e_slot, dErrs.as<double>(), nn, 16, dDots.as<double>(), S(ctx)));
-            double hd[16 + 1];
+            double hd[16 + 1], he[33];
             HIPCHK(hipMemcpyAsync(hd, dDots.p, 16 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipMemcpyAsync(hd + 16, scal, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            if (ediis_on) HIPCHK(hipMemcpyAsync(he, dEd.p, 33 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
             for (int i = 0; i < m; ++i) newdots[i] = hd[slots[i]];
             out->last_error = hd[16];
+            if (ediis_on) {  // the new row (P_new F_j) and column (P_j F_new) of tr(P_i F_j)
+                for (int i = 0; i < m - 1; ++i) pf[i].push_back(he[16 + slots[i]]);
+                std::vector<double> row(m);
+                for (int j = 0; j < m; ++j) row[j] = he[slots[j]];
+                pf.push_back(row);
+                ph.push_back(he[32]);
+            }
         }
         const double err = out->last_error;
         if (!is_normal(err) && !(err < opts.epsilon)) {  // NaN or inf: nothing below is meaningful
@@ -3044,7 +3173,26 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         const double *Fuse = f_slot;
         const bool exit_plain = opts.integral_policy == FOCK_POLICY_ACCURATE && phase == 1 && !incr_build &&
                                 err < opts.epsilon;
-        if (m >= opts.diis_start && !exit_plain) {
+        bool extrapolated = false;
+        if (ediis_on && m >= 2 && err > opts.ediis_until && !exit_plain) {
+            // EDIIS: e_i = 1/2 (tr(P_i H) + tr(P_i F_i)), Q = -1/2 D with
+            // D_ij = tr((P_i - P_j)(F_i - F_j)) = a_ii + a_jj - a_ij - a_ji, a_ij = tr(P_i F_j)
+            std::vector<double> ev(m), Q((size_t)m * m), c;
+            for (int i = 0; i < m; ++i) ev[i] = 0.5 * (ph[i] + pf[i][i]);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j)
+                    Q[(size_t)i * m + j] = -0.5 * (pf[i][i] + pf[j][j] - pf[i][j] - pf[j][i]);
+            if (simplex_qp(m, ev, Q, c)) {
+                std::vector<int> sl(slots.begin(), slots.end());
+                HIPCHK(hipMemcpyAsync(dCoef.p, c.data(), m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+                HIPCHK(hipMemcpyAsync(dSlots.p, sl.data(), m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+                HIPCHK(lincomb(nn, dFocks.as<double>(), nn, dSlots.as<int>(), dCoef.as<double>(), m, Fx_, S(ctx)));
+                HIPCHK(hipStreamSynchronize(S(ctx)));  // c, sl are host temporaries
+                Fuse = Fx_;
+                extrapolated = true;
+            }
+        }
+        if (m >= opts.diis_start && !exit_plain && !extrapolated) {
             std::vector<double> Bm((size_t)m * m), c;
             for (int i = 0; i < m; ++i)
                 for (int j = 0; j < m; ++j) Bm[(size_t)i * m + j] = dots[i][j];
@@ -3070,7 +3218,11 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         // over to an FP64 full build before returning)
         const bool returnable = phase == 1 && !incr_build;
         const bool may_exit = (returnable && err < opts.epsilon) || iter >= opts.max_iters - 1 || !is_normal(err);
+        const bool damp = damping_on && err > opts.damping_until;
+        if (damp) HIPCHK(copy_f64(nn, P_, dPold.as<double>(), S(ctx)));
         STCHK(diagonalize(Fuse, may_exit));
+        // P = F P_new + (1 - F) P_old (scf/src/lib.rs:218-219, F = opts.damping)
+        if (damp) HIPCHK(mix_f64(nn, opts.damping, P_, dPold.as<double>(), P_, S(ctx)));
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
         HIPCHK(energy_trace(n, P_, H_, G_, scal + 1, S(ctx)));
         double e_elec = 0.0;

@@ -341,6 +341,8 @@ hipError_t energy_trace(int n, const double *P, const double *H, const double *G
 // x = a - b elementwise (len)
 hipError_t sub_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s);
 hipError_t add_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s);
+// out = lam a + (1 - lam) b (density damping)
+hipError_t mix_f64(int64_t len, double lam, const double *a, const double *b, double *out, hipStream_t s);
 hipError_t transpose_f64(int n, const double *in, double *out, hipStream_t s);
 hipError_t fill_f64(int64_t len, double v, double *out, hipStream_t s);
 hipError_t copy_f64(int64_t len, const double *in, double *out, hipStream_t s);

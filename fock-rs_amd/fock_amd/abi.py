@@ -68,7 +68,8 @@ class ScfOpts(ctypes.Structure):
                 ("incremental", ctypes.c_int32), ("initial_density", _dp), ("diis_start", ctypes.c_int32),
                 ("diis_window", ctypes.c_int32), ("level_shift", ctypes.c_double),
                 ("level_shift_until", ctypes.c_double), ("autotune", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("ediis", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2), ("ediis_until", ctypes.c_double),
+                ("damping", ctypes.c_double), ("damping_until", ctypes.c_double)]
 
 
 class ScfIter(ctypes.Structure):

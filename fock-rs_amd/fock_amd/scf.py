@@ -122,19 +122,24 @@ def try_scf(atoms: Iterable[Atom], max_iters: int, epsilon: float, molecule_char
             incremental: bool = False, ctx: Optional[abi.Context] = None,
             initial_density: Optional[np.ndarray] = None, diis_start: int = 5, diis_window: int = 16,
             level_shift: float = 0.0, level_shift_until: float = 0.0,
-            autotune: bool = False) -> Optional[HartreeFockResult]:
+            autotune: bool = False, ediis: bool = False, ediis_until: float = 0.1,
+            damping: float = 1.0, damping_until: float = 0.0) -> Optional[HartreeFockResult]:
     """SelfConsistentField::try_scf (scf/src/lib.rs:73-258) on the GPU.
 
     Defaults reproduce the reference's own arithmetic and schedule (REF_COMPAT integrals, its
     1e-6 basis-function-pair screen, FP64, GWH guess, DIIS from 5 stored matrices in a window
-    of 16). autotune: time the J/K tiles at the guess density first (kept in the context)."""
+    of 16). autotune: time the J/K tiles at the guess density first (kept in the context).
+    ediis / damping: EDIIS above an error of ediis_until, density mixing factor (the reference's F of
+    scf/src/lib.rs:218, fixed there at 1) above damping_until."""
     atoms = list(atoms)
     fb, fn = flatten(atoms)
     opts = abi.default_opts(max_iters=max_iters, epsilon=epsilon, molecule_charge=molecule_charge,
                             integral_policy=policy, screen=screen, schwarz_threshold=schwarz_threshold,
                             precision=precision, incremental=1 if incremental else 0,
                             diis_start=diis_start, diis_window=diis_window, level_shift=level_shift,
-                            level_shift_until=level_shift_until, autotune=1 if autotune else 0)
+                            level_shift_until=level_shift_until, autotune=1 if autotune else 0,
+                            ediis=1 if ediis else 0, ediis_until=ediis_until, damping=damping,
+                            damping_until=damping_until)
     if initial_density is not None:
         P0 = np.ascontiguousarray(initial_density, dtype=np.float64)
         if P0.shape != (fb.n, fb.n):

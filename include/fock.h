@@ -123,7 +123,19 @@ typedef struct fock_scf_opts {
      * first iteration's density before iterating; the chosen tiles stay in the context for later
      * calls. 0 (default): the context's current tiles. */
     int32_t autotune;
-    int32_t reserved[3];
+    /* Nonzero: while max|FDS - SDF| > ediis_until, fock_scf_run extrapolates with EDIIS (energy
+     * DIIS, Kudin, Scuseria and Cances, J. Chem. Phys. 116, 8255 (2002)) instead of DIIS: the
+     * convex combination sum c_i F_i (c_i >= 0, sum c_i = 1) of the stored Fock matrices that
+     * minimises the interpolated energy sum c_i E_i - 1/4 sum c_i c_j tr((P_i - P_j)(F_i - F_j)),
+     * E_i = 1/2 tr(P_i (H + F_i)); DIIS below ediis_until. Single-molecule SCF only. */
+    int32_t ediis;
+    int32_t reserved[2];
+    double ediis_until;
+    /* Density damping: P <- damping P_new + (1 - damping) P_old while max|FDS - SDF| >
+     * damping_until, the mixing of scf/src/lib.rs:218-219 whose factor the reference fixes at 1.
+     * Values <= 0 or >= 1 mean no damping (the default). Single-molecule SCF only. */
+    double damping;
+    double damping_until;
 } fock_scf_opts;
 
 /* Per-iteration record (optional output). */

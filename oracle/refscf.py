@@ -148,6 +148,37 @@ def diis(errors, focks) -> Optional[np.ndarray]:
     return F
 
 
+def ediis_coefficients(e: np.ndarray, Q: np.ndarray) -> Optional[np.ndarray]:
+    """min e.c + 1/2 c^T Q c over the simplex (c >= 0, sum c = 1) by enumeration: for every subset
+    of free indices, the equality-constrained optimum [Q_FF 1; 1^T 0][c_F; lam] = [-e_F; 1]; the
+    lowest objective among the feasible ones (c_F >= 0) is the minimum over the simplex when the
+    objective is convex there. Exact and independent of the library's active-set method; for the
+    small windows of the tests (m <= 10: 1023 subsets)."""
+    m = len(e)
+    best, best_val = None, np.inf
+    for mask in range(1, 1 << m):
+        F = [i for i in range(m) if mask >> i & 1]
+        f = len(F)
+        A = np.zeros((f + 1, f + 1))
+        A[:f, :f] = Q[np.ix_(F, F)]
+        A[:f, f] = 1.0
+        A[f, :f] = 1.0
+        rhs = np.concatenate([-e[F], [1.0]])
+        try:
+            x = np.linalg.solve(A, rhs)
+        except np.linalg.LinAlgError:
+            continue
+        if np.any(x[:f] < -1e-12) or not np.all(np.isfinite(x)):
+            continue
+        c = np.zeros(m)
+        c[F] = np.maximum(x[:f], 0.0)
+        c /= c.sum()
+        val = float(e @ c + 0.5 * c @ Q @ c)
+        if val < best_val - 1e-15:
+            best, best_val = c, val
+    return best
+
+
 def density(C: np.ndarray, n_electrons: int) -> np.ndarray:
     occ = C[:, : n_electrons // 2]
     return 2.0 * occ @ occ.T
@@ -156,7 +187,8 @@ def density(C: np.ndarray, n_electrons: int) -> np.ndarray:
 def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, accurate: bool = False,
             screen: bool = True, integrals=None, P0=None, diis_start: int = 5, diis_window: int = 16,
             level_shift: float = 0.0, level_shift_until: float = 0.0,
-            exit_plain: Optional[bool] = None) -> Optional[ScfResult]:
+            exit_plain: Optional[bool] = None, damping: float = 1.0, damping_until: float = 0.0,
+            ediis: bool = False, ediis_until: float = 0.1) -> Optional[ScfResult]:
     """scf/src/lib.rs:73-258 on the flat basis `fb` and nuclei `fn`.
 
     accurate=False: the reference's own integrals (refcpu.c, LUT Boys, x-branch as written,
@@ -168,6 +200,9 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
     the newest Fock matrix instead of the DIIS extrapolation -- the rule the GPU driver applies
     under its accurate policy (an extrapolation from parallel errors is a singular system whose
     solution follows rounding); the reference's own rule is exit_plain=False.
+    damping: the mixing factor F of scf/src/lib.rs:218-219 (the reference fixes F = 1), applied
+    while the error exceeds damping_until. ediis: while the error exceeds ediis_until, the Fock
+    matrix is the EDIIS combination of the stored ones (ediis_coefficients) instead of DIIS.
     """
     if exit_plain is None:
         exit_plain = accurate
@@ -194,7 +229,7 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
         C = X @ Cp
         P = density(C, n_electrons)
     gbuild = GBuilder(n, packed)
-    focks, errs = deque(), deque()
+    focks, errs, dens = deque(), deque(), deque()
     err_hist, e_hist = [], []
     for it in range(max_iters + 1):
         G = gbuild(P)
@@ -203,10 +238,24 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
         err = float(np.max(np.abs(E)))
         errs.append(E)
         focks.append(F)
+        dens.append(P)
         if len(errs) > diis_window:
             errs.popleft()
             focks.popleft()
-        if len(focks) < diis_start or (exit_plain and err < epsilon):
+            dens.popleft()
+        Fx = None
+        if ediis and len(focks) >= 2 and err > ediis_until and not (exit_plain and err < epsilon):
+            m_ = len(focks)
+            a = np.array([[float(np.sum(dens[i] * focks[j])) for j in range(m_)] for i in range(m_)])
+            e = np.array([0.5 * (float(np.sum(dens[i] * H)) + a[i, i]) for i in range(m_)])
+            d = np.diag(a)
+            Q = -0.5 * (d[:, None] + d[None, :] - a - a.T)
+            c = ediis_coefficients(e, Q)
+            if c is not None:
+                Fx = sum(ci * Fi for ci, Fi in zip(c, focks))
+        if Fx is not None:
+            pass
+        elif len(focks) < diis_start or (exit_plain and err < epsilon):
             Fx = focks[-1]
         else:
             Fx = diis(errs, focks)
@@ -216,7 +265,9 @@ def try_scf(fb, fn, max_iters: int, epsilon: float, molecule_charge: int = 0, ac
             Fx = Fx + level_shift * (S - 0.5 * (S @ P) @ S)
         Cp, eps = sorted_eigs(X.T @ (Fx @ X))
         C = X @ Cp
-        P = density(C, n_electrons)
+        P_new = density(C, n_electrons)
+        # new_density = F new + (1 - F) density (scf/src/lib.rs:218-219)
+        P = damping * P_new + (1.0 - damping) * P if (0.0 < damping < 1.0 and err > damping_until) else P_new
         e_elec = 0.5 * float(np.trace(P @ (2.0 * H + G)))
         err_hist.append(err)
         e_hist.append(e_elec + e_nuc)

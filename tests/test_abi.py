@@ -37,13 +37,14 @@ def test_default_options_are_the_reference_constants():
     assert o.precision == abi.PRECISION_FP64 and o.incremental == 0
     assert o.diis_start == 5 and o.diis_window == 16 and o.level_shift == 0.0  # scf/src/lib.rs:197-207
     assert o.autotune == 0
+    assert o.ediis == 0 and o.damping == 1.0          # the reference's mixing factor F = 1.0 (:218)
 
 
 def test_struct_sizes_match_header_layout():
     # offsets of the C structs (x86-64 SysV): keeps the ctypes mirror honest
     assert ctypes.sizeof(abi.BasisDesc) == 48
     assert ctypes.sizeof(abi.NucleiDesc) == 24
-    assert ctypes.sizeof(abi.ScfOpts) == 96
+    assert ctypes.sizeof(abi.ScfOpts) == 120
     assert ctypes.sizeof(abi.ScfIter) == 32
     assert ctypes.sizeof(abi.JKStats) == 12 + 4 + 6 * 8 * 5 + 8 + 8 + 8
     assert ctypes.sizeof(abi.TuneReport) == 6 * 4 + 6 * 5 * 8 + 4 + 4 + 2 * 8 + 8

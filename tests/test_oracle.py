@@ -444,3 +444,28 @@ def test_refmo_truncation_and_voxel_order():
         x, y, z = i % N, (i // N) % N, i // N ** 2
         assert np.array_equal(pts[i], np.array([x, y, z], dtype=np.float64) / N * box - 0.5 * box) or \
             np.abs(pts[i] - (np.array([x, y, z]) / N - 0.5) * box).max() < 1e-14
+
+
+def test_ediis_coefficients_satisfy_the_simplex_kkt_conditions():
+    """oracle.refscf.ediis_coefficients on random convex instances (Q = -1/2 D with D_ij =
+    |x_i - x_j|^2, conditionally negative definite, as tr((P_i - P_j)(F_i - F_j)) is): the result
+    is on the simplex and certifies optimality -- the gradient g = e + Q c is equal on the support
+    and no smaller elsewhere -- and no random simplex point does better."""
+    from oracle import refscf
+    rng = np.random.default_rng(4)
+    for m_ in (2, 3, 5, 8):
+        for _ in range(5):
+            x = rng.standard_normal((m_, 3))
+            D = np.sum((x[:, None] - x[None]) ** 2, axis=-1)
+            Q = -0.5 * D
+            e = rng.standard_normal(m_) * 2.0
+            c = refscf.ediis_coefficients(e, Q)
+            assert c is not None and np.all(c >= 0) and abs(c.sum() - 1.0) < 1e-12
+            g = e + Q @ c
+            sup = c > 1e-10
+            assert np.ptp(g[sup]) < 1e-9 * max(1.0, np.abs(g).max())
+            assert np.all(g[~sup] >= g[sup].min() - 1e-9)
+            f = lambda v: e @ v + 0.5 * v @ Q @ v
+            for _ in range(50):
+                v = rng.dirichlet(np.ones(m_))
+                assert f(v) >= f(c) - 1e-12
