@@ -154,6 +154,7 @@ def batch_form(ctx, abi, m, np):
         "workload": "64 (H2O)_k clusters, k in 1..8, 6-31G (n = 13..104), accurate, Schwarz 1e-12",
         "status": abi.STATUS_NAMES[st], "batch_wall_s": t_batch, "sequential_wall_s": t_seq,
         "speedup": t_seq / t_batch, "scf_per_s": 64 / t_batch,
+        "iterations_max": int(max(r["iterations"] for r in res)),
         "max_abs_dE_batch_vs_single": float(max(abs(a["total_energy"] - b["total_energy"]) for a, b in zip(res, single))),
         "n_basis_range": [min(fb.n for fb, _ in mols), max(fb.n for fb, _ in mols)],
         "tiles": ctx.tiles(abi.TILES_BATCH)[0],

@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void gemm_f64_batch_kernel(bool transA, bool t
     const GemmRec r = recs[blockIdx.z];
     const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
     if (row0 >= r.m || col0 >= r.n) return;  // the same for the whole workgroup
+    if (r.flag && *r.flag == 0) return;      // the same for the whole workgroup
     gemm_f64_tile(transA, transB, r.m, r.n, r.k, alpha, r.A, r.lda, r.B, r.ldb, beta, r.C, r.ldc, row0, col0);
 }
 
@@ -479,6 +480,7 @@ __global__ __launch_bounds__(256) void jacobi_eig_kernel(const EigRec *__restric
     const EigRec r = recs[blockIdx.x];
     const int n = r.n;
     if (n <= 0 || n > JACOBI_MAX_N) return;  // the same for the whole workgroup
+    if (r.flag && *r.flag == 0) return;      // the same for the whole workgroup
     extern __shared__ double U[];  // [n][n]: column k at U[k * n]
     __shared__ double s_red[4];
     __shared__ double s_norm[JACOBI_MAX_N];
@@ -588,6 +590,7 @@ __global__ void scatter_jk_batch_kernel(const ScatterRec *__restrict__ recs) {
     const ScatterRec r = recs[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= r.npair) return;
+    if (r.flag && *r.flag == 0) return;
     const int2 ab = r.pair_shell[j];
     const int fa = r.shell_f0[ab.x], fb = r.shell_f0[ab.y], na = r.shell_nf[ab.x], nb = r.shell_nf[ab.y];
     for (int c = 0; c < na; ++c)

@@ -474,11 +474,14 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         if (bb.x < 0) return;  // the same for every thread of the block
         const BatchRec r = args_in.batch[bb.x];
         if (!r.active) return;  // the same for every thread of the block
+        // device-driven iteration: the molecule still iterates, in this kernel's precision
+        if (r.state && (!r.state->active || r.state->phase != (sizeof(Real) == 4 ? 0 : 1))) return;
         args.tasks = r.tasks;
         args.pairs = r.pairs;
-        args.prims = r.prims;
+        args.prims = sizeof(Real) == 4 ? r.prims32 : r.prims;
         args.prim_q = r.prim_q;
-        args.soa = r.soa;
+        args.soa = sizeof(Real) == 4 ? r.soa32 : r.soa;
+        if constexpr (sizeof(Real) == 4) args.pairQ = r.pairQ;  // the shell-quartet cut (ket_tau)
         args.soa_pairs = r.soa_pairs;
         args.soa_stride = r.soa_stride;
         args.D = r.D;
@@ -741,7 +744,11 @@ static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 static hipError_t dispatch_block_policy(const JKArgs &a, int policy, bool refscreen, bool fp32, hipStream_t s) {
     if (fp32) {
-        if (policy != 1 || a.batch) return hipErrorInvalidValue;  // batches run the FP64 form
+        if (policy != 1) return hipErrorInvalidValue;
+        if (a.batch) {  // mixed-form batches: accurate policy, Schwarz screen or none
+            if (refscreen) return hipErrorInvalidValue;
+            return launch_block<LA, LB, LC, LD, 1, false, float, true>(a, s);
+        }
         return refscreen ? launch_block<LA, LB, LC, LD, 1, true, float>(a, s)
                          : launch_block<LA, LB, LC, LD, 1, false, float>(a, s);
     }

@@ -2029,6 +2029,12 @@ fock_status eig_batch(fock_ctx *ctx, int n, int count, double *A, double *W, dou
     return FOCK_OK;
 }
 
+// The device-driven batch iteration (every step of an iteration in one captured graph, the
+// DIIS ring, the exit rule and the mixed form's phase kept in per-molecule states on the device)
+// is switched off: its first run on the GPU ended in a GPU memory fault whose cause has not been
+// found. Batches run the host-driven loop (FP64 form) until it is.
+constexpr bool DEVICE_DRIVEN_BATCH = false;
+
 }  // namespace
 
 extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fock_basis_desc *bases,
@@ -2039,8 +2045,12 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     fock_scf_opts opts;
     if (opts_in) opts = *opts_in;
     else fock_scf_opts_default(&opts);
-    if (opts.precision != FOCK_PRECISION_FP64 || opts.incremental)
-        return ctx->fail(FOCK_UNSUPPORTED, "the batched SCF runs the FP64 form");
+    const bool mixed = opts.precision == FOCK_PRECISION_MIXED;
+    if (opts.precision != FOCK_PRECISION_FP64 && !mixed) return ctx->fail(FOCK_INVALID_ARG, "unknown precision");
+    if (opts.incremental) return ctx->fail(FOCK_UNSUPPORTED, "no incremental builds in the batched SCF");
+    if (mixed && !DEVICE_DRIVEN_BATCH) return ctx->fail(FOCK_UNSUPPORTED, "the batched SCF runs the FP64 form");
+    if (mixed && (opts.integral_policy != FOCK_POLICY_ACCURATE || opts.screen == FOCK_SCREEN_REF_BF_PAIR))
+        return ctx->fail(FOCK_UNSUPPORTED, "the mixed batched SCF needs the accurate policy and the Schwarz screen or none");
     if (opts.initial_density) return ctx->fail(FOCK_UNSUPPORTED, "no initial density in the batched SCF");
     if (opts.ediis || (opts.damping > 0.0 && opts.damping < 1.0))
         return ctx->fail(FOCK_UNSUPPORTED, "EDIIS and damping are options of the single-molecule SCF");
@@ -2108,7 +2118,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         return FOCK_OK;
     };
     STCHK(for_molecules([&](BatchMol &b) -> fock_status {
-        return prepare_into(ctx, b.pr, b.basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, 0);
+        return prepare_into(ctx, b.pr, b.basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0);
     }));
     phase_mark("prepare");
     if (!blocked_fits(ctx, nmax, false, opts.integral_policy, ctx->tiles[FOCK_TILES_BATCH]))
@@ -2198,6 +2208,23 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     // LDS-resident Jacobi solver when every basis fits it, else one rocSOLVER batched call per
     // size group (a group whose molecules have all converged is skipped).
     const bool use_jacobi = nmax <= JACOBI_MAX_N;
+    // device-driven iterations (one captured graph and one host synchronisation per iteration) when
+    // every eigenproblem fits the batched Jacobi solver; otherwise the host-driven loop below with
+    // rocSOLVER per size group (FP64 form)
+    const bool device_driven = DEVICE_DRIVEN_BATCH && use_jacobi;
+    if (mixed && !device_driven) return ctx->fail(FOCK_UNSUPPORTED, "the mixed batched SCF needs bases of at most 128 functions");
+    DevBuf d_states;
+    if (device_driven) {
+        HIPCHK(d_states.ensure((size_t)count * sizeof(BatchState)));
+        std::vector<BatchState> h(count);
+        for (auto &x : h) {
+            std::memset(&x, 0, sizeof(x));
+            x.active = 1;
+            x.phase = 1;  // FP64 kernels while tiles are tuned; the mixed form starts in phase 0 below
+        }
+        HIPCHK(hipMemcpy(d_states.p, h.data(), count * sizeof(BatchState), hipMemcpyHostToDevice));
+    }
+    BatchState *states = d_states.as<BatchState>();
     auto eig_all = [&](bool only_active) -> fock_status {
         if (use_jacobi) {
             std::vector<EigRec> erec(count);
@@ -2400,6 +2427,10 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
                 r.npair = (int)pr.pairs.size();
                 r.nbf = pr.n;
                 r.active = mols[m]->done ? 0 : 1;
+                r.state = states ? states + m : nullptr;
+                r.prims32 = pr.d_prims32.p;
+                r.soa32 = pr.d_soa32.p;
+                r.pairQ = pr.d_pairQ.as<float>();
             }
         }
         HIPCHK(hipMemcpyAsync(d_recs.p, recs.data(), recs.size() * sizeof(BatchRec), hipMemcpyHostToDevice, st));
@@ -2473,6 +2504,197 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
     STCHK(upload_records());
     for (int c = 0; c < NUM_CLASSES; ++c) STCHK(make_schedule(c));
     phase_mark("guess_tables");
+    if (device_driven) {
+        // ---- one iteration = one graph: the build, the DIIS error and solve, the level shift, the
+        // eigenproblems, the density, the energy and the exit rule of every molecule, controlled by
+        // the BatchStates; the host reads the number of molecules still iterating once per iteration
+        std::vector<BatchState> hs(count);
+        for (int m = 0; m < count; ++m) {
+            BatchState &x = hs[m];
+            std::memset(&x, 0, sizeof(x));
+            x.active = mols[m]->done ? 0 : 1;
+            x.status = mols[m]->done ? mols[m]->status : FOCK_NOT_CONVERGED;
+            x.phase = mixed ? 0 : 1;
+            x.best_err = HUGE_VAL;
+        }
+        HIPCHK(hipMemcpyAsync(states, hs.data(), count * sizeof(BatchState), hipMemcpyHostToDevice, st));
+        for (int m = 0; m < count; ++m) scat[m].flag = &states[m].active;
+        HIPCHK(hipMemcpyAsync(d_scat.p, scat.data(), count * sizeof(ScatterRec), hipMemcpyHostToDevice, st));
+        std::vector<BatchMolRec> mrec(count);
+        enum { G_FP, G_FPS, G_SP, G_SPS, G_FX, G_XF, G_FE, G_EF, G_EM, G_C, G_P, G_NUM };
+        std::vector<GemmRec> gr_all((size_t)G_NUM * count);
+        std::vector<EigRec> erec(count);
+        for (int m = 0; m < count; ++m) {
+            BatchMol &b = *mols[m];
+            const int n = b.n;
+            const int *act = &states[m].active, *sh = &states[m].shift;
+            double *T1 = b.T1.as<double>(), *T2 = b.T2.as<double>();
+            mrec[m] = BatchMolRec{n, states + m, b.pr.d_Jb.as<double>(), b.pr.d_Kb.as<double>(), b.H.as<double>(),
+                                  b.S.as<double>(), b.G.as<double>(), b.F.as<double>(), b.Fx.as<double>(), T2,
+                                  b.errs.as<double>(), b.focks.as<double>(), b.U.as<double>(), b.E.as<double>(),
+                                  b.P.as<double>()};
+            GemmRec *g = gr_all.data() + m;
+            auto put = [&](int k, GemmRec r) { g[(size_t)k * count] = r; };
+            put(G_FP, GemmRec{n, n, n, n, n, n, b.F.as<double>(), b.P.as<double>(), T1, act});       // F P
+            put(G_FPS, GemmRec{n, n, n, n, n, n, T1, b.S.as<double>(), T2, act});                     // (F P) S
+            put(G_SP, GemmRec{n, n, n, n, n, n, b.S.as<double>(), b.P.as<double>(), T1, sh});         // S P
+            put(G_SPS, GemmRec{n, n, n, n, n, n, T1, b.S.as<double>(), T2, sh});                      // (S P) S
+            put(G_FX, GemmRec{n, n, n, n, n, n, b.Fx.as<double>(), b.X.as<double>(), T1, act});      // F X
+            put(G_XF, GemmRec{n, n, n, n, n, n, b.X.as<double>(), T1, gA(b), act});                  // X^T (F X)
+            put(G_FE, GemmRec{n, n, n, n, n, n, gA(b), b.E.as<double>(), T2, act});                  // F' E^T
+            put(G_EF, GemmRec{n, n, n, n, n, n, b.E.as<double>(), T2, gA(b), act});                  // E (F' E^T)
+            put(G_EM, GemmRec{n, n, n, n, n, n, gA(b), b.E.as<double>(), b.U.as<double>(), act});    // E_M E
+            put(G_C, GemmRec{n, n, n, n, n, n, b.X.as<double>(), b.E.as<double>(), b.C.as<double>(), act});  // X E^T
+            put(G_P, GemmRec{n, n, b.nocc, n, n, n, b.C.as<double>(), b.C.as<double>(), b.P.as<double>(), act});
+            erec[m] = EigRec{n, gA(b), gW(b), act};
+        }
+        DevBuf d_mrec, d_gr, d_eig, d_count;
+        HIPCHK(d_mrec.ensure(count * sizeof(BatchMolRec)));
+        HIPCHK(d_gr.ensure(gr_all.size() * sizeof(GemmRec)));
+        HIPCHK(d_eig.ensure(count * sizeof(EigRec)));
+        HIPCHK(d_count.ensure(sizeof(int)));
+        HIPCHK(hipMemcpyAsync(d_mrec.p, mrec.data(), count * sizeof(BatchMolRec), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_gr.p, gr_all.data(), gr_all.size() * sizeof(GemmRec), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_eig.p, erec.data(), count * sizeof(EigRec), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));  // host record arrays go out of scope after the loop only
+        const BatchMolRec *MR = d_mrec.as<BatchMolRec>();
+        auto grec = [&](int k) { return d_gr.as<GemmRec>() + (size_t)k * count; };
+        BatchParams bp{};
+        bp.max_iters = opts.max_iters;
+        bp.diis_start = opts.diis_start;
+        bp.diis_window = opts.diis_window;
+        bp.accurate = opts.integral_policy == FOCK_POLICY_ACCURATE ? 1 : 0;
+        bp.mixed = mixed ? 1 : 0;
+        bp.eps = opts.epsilon;
+        bp.level_shift = opts.level_shift;
+        bp.level_shift_until = opts.level_shift_until;
+        bp.mixed_switch = 1e-4;
+        bp.mixed_stall = 1e-2;
+        int *dcount = d_count.as<int>();
+        // the FP32 class kernels of the mixed form (blocks of molecules in phase 1 return at once,
+        // and the FP64 kernels skip the molecules in phase 0)
+        auto launch_class32 = [&](int cls) -> fock_status {
+            if (bbl[cls].empty()) return FOCK_OK;
+            JKArgs a{};
+            a.nbf = nmax;
+            a.nblocks = (int)bbl[cls].size();
+            a.tile = tiles[cls];
+            a.bt = ctx->bt;
+            a.prim_thr = PRIM_THRESHOLD32;
+            a.ket_tau = PRIM_THRESHOLD32;
+            a.batch = d_recs.as<BatchRec>() + 0;
+            a.bblocks = d_bbl[cls].as<int2>();
+            if (block_lds_bytes(cls, nmax, FOCK_POLICY_ACCURATE, true, tiles[cls]) > ctx->max_lds)
+                return ctx->fail(FOCK_UNSUPPORTED, "FP32 batch workgroup exceeds the LDS");
+            HIPCHK(launch_eri_blocks(cls, a, FOCK_POLICY_ACCURATE, false, true, st));
+            return FOCK_OK;
+        };
+        auto enqueue_iteration = [&]() -> fock_status {
+            HIPCHK(batch_begin(MR, count, nmax, dcount, st));
+            for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+                STCHK(launch_class(cls));
+                if (mixed) STCHK(launch_class32(cls));
+            }
+            HIPCHK(scatter_jk_batch(count, d_scat.as<ScatterRec>(), max_npair, st));
+            HIPCHK(batch_fock_from_jk(MR, count, nmax, st));
+            // DIIS error F P S - S P F (scf/src/lib.rs:189-192) and the extrapolation (:194-207)
+            HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_FP), count, nmax, nmax, st));
+            HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_FPS), count, nmax, nmax, st));
+            HIPCHK(batch_diis_error(MR, count, nmax, st));
+            HIPCHK(batch_diis_update(MR, count, bp, st));
+            HIPCHK(batch_lincomb(MR, count, nmax, st));
+            if (opts.level_shift > 0.0) {  // virtual orbitals raised by sigma: F + sigma (S - S P S / 2)
+                HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_SP), count, nmax, nmax, st));
+                HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_SPS), count, nmax, nmax, st));
+                HIPCHK(batch_level_shift(MR, count, nmax, opts.level_shift, st));
+            }
+            // F' = X^T F X in the basis E of the previous eigenvectors, eigenproblems, C = X C', P
+            // (scf/src/lib.rs:210-216)
+            HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_FX), count, nmax, nmax, st));
+            HIPCHK(gemm_f64_batch(true, false, 1.0, 0.0, grec(G_XF), count, nmax, nmax, st));
+            HIPCHK(gemm_f64_batch(false, true, 1.0, 0.0, grec(G_FE), count, nmax, nmax, st));
+            HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_EF), count, nmax, nmax, st));
+            HIPCHK(jacobi_eig_batch(d_eig.as<EigRec>(), count, nmax, st));
+            HIPCHK(gemm_f64_batch(false, false, 1.0, 0.0, grec(G_EM), count, nmax, nmax, st));
+            HIPCHK(batch_copy_basis(MR, count, nmax, st));
+            HIPCHK(gemm_f64_batch(false, true, 1.0, 0.0, grec(G_C), count, nmax, nmax, st));
+            HIPCHK(gemm_f64_batch(false, true, 2.0, 0.0, grec(G_P), count, nmax, nmax, st));
+            // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223) and the exit rule (:224-249)
+            HIPCHK(batch_energy(MR, count, st));
+            HIPCHK(batch_exit(MR, count, bp, dcount, st));
+            return FOCK_OK;
+        };
+        struct GraphGuard {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            ~GraphGuard() {
+                if (x) (void)hipGraphExecDestroy(x);
+                if (g) (void)hipGraphDestroy(g);
+            }
+        } graph;
+        if (st != nullptr && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            const fock_status cs = enqueue_iteration();
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (cs == FOCK_OK && ce == hipSuccess && g) {
+                graph.g = g;
+                if (hipGraphInstantiate(&graph.x, g, nullptr, nullptr, 0) != hipSuccess) graph.x = nullptr;
+            } else if (g) {
+                (void)hipGraphDestroy(g);
+            }
+            (void)hipGetLastError();  // a failed capture leaves no launch behind; fall back to eager
+        }
+        struct Pinned {
+            int *h = nullptr;
+            ~Pinned() {
+                if (h) (void)hipHostFree(h);
+            }
+        } pin;
+        HIPCHK(hipHostMalloc((void **)&pin.h, sizeof(int), hipHostMallocDefault));
+        phase_mark("graph");
+        for (int iter = 0; iter <= opts.max_iters; ++iter) {
+            if (graph.x) HIPCHK(hipGraphLaunch(graph.x, st));
+            else STCHK(enqueue_iteration());
+            HIPCHK(hipMemcpyAsync(pin.h, dcount, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));  // the one host synchronisation of the iteration
+            if (debug) std::fprintf(stderr, "[fock-batch] iter %d active %d\n", iter, *pin.h);
+            if (*pin.h == 0) break;
+        }
+        phase_mark("iterations");
+        HIPCHK(hipMemcpy(hs.data(), states, count * sizeof(BatchState), hipMemcpyDeviceToHost));
+        fock_status result = FOCK_OK;
+        const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        for (int m = 0; m < count; ++m) {
+            BatchMol &b = *mols[m];
+            const BatchState &x = hs[m];
+            int32_t stm = x.active ? FOCK_NOT_CONVERGED : x.status;
+            fock_scf_out *o = b.out;
+            o->scf_wall_ms = wall;
+            o->history_len = 0;
+            if (status) status[m] = stm;
+            if (stm == FOCK_DIVERGED) {
+                result = FOCK_DIVERGED;
+                continue;
+            }
+            if (stm != FOCK_OK && result == FOCK_OK) result = FOCK_NOT_CONVERGED;
+            const int n = b.n;
+            const size_t bytes = (size_t)n * n * sizeof(double);
+            o->electronic_energy = x.e_elec;
+            o->total_energy = x.e_elec + b.e_nuc;
+            o->iterations = x.iter_done;
+            o->last_error = x.err;
+            if (o->coefficients) {  // column-major (nalgebra layout)
+                HIPCHK(transpose_f64(n, b.C.as<double>(), b.T1.as<double>(), st));
+                HIPCHK(hipMemcpyAsync(o->coefficients, b.T1.p, bytes, hipMemcpyDeviceToHost, st));
+            }
+            if (o->density) HIPCHK(hipMemcpyAsync(o->density, b.P.p, bytes, hipMemcpyDeviceToHost, st));
+            if (o->orbital_energies)
+                HIPCHK(hipMemcpyAsync(o->orbital_energies, gW(b), n * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        if (result != FOCK_OK) ctx->err = "some molecules of the batch did not converge";
+        return result;
+    }
     std::vector<double> hostd((size_t)count * 18);
     int active = count - n_singular;
     // G(P), F = H + G of the whole batch: one launch per quartet class; converged molecules'

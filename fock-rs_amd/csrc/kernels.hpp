@@ -67,6 +67,28 @@ struct JKArgs {
     const struct BatchRec *batch;
     const int2 *bblocks;
 };
+// Iteration state of one molecule of the device-driven batched SCF: read and written by the kernels
+// of the iteration graph (the host reads it after the last iteration).
+struct BatchState {
+    int active;          // 1 while the molecule iterates; every kernel of the iteration skips it at 0
+    int status;          // fock_status of the exit
+    int iter;            // index of the iteration being run
+    int iter_done;       // index of the returning iteration
+    int ring_len, next_slot;
+    int ring[16];        // ring slots of the stored (error, Fock) pairs, oldest first
+    int ncoef;           // the Fock matrix of this iteration: sum_i coef[i] focks[cslot[i]]
+    int cslot[16];
+    int shift;           // 1: the level shift applies this iteration
+    int phase;           // 0: FP32 quartet integrals (mixed form), 1: FP64
+    int stall;
+    int pad_;
+    double coef[16];
+    double best_err;
+    double err, e_elec;
+    double maxabs;       // max |FPS - SPF| of this iteration (accumulated by the error kernel)
+    double dots[16];     // <e_new, e_slot> by ring slot
+    double B[16 * 16];   // <e_i, e_j> of the stored pairs in ring order
+};
 // one molecule's arrays for one quartet class of a batched Fock build
 struct BatchRec {
     const Task *tasks;
@@ -82,6 +104,11 @@ struct BatchRec {
     double *J, *K, *Jk;
     int npair, nbf;
     int active;  // 0: the molecule has converged, its blocks return at once
+    // device-driven iteration: the blocks of a molecule run only while state->active and when the
+    // molecule's phase matches the kernel's precision (FP32 kernels: phase 0); null: host-driven
+    const BatchState *state;
+    const void *prims32, *soa32;  // FP32 copies of prims and soa (mixed form)
+    const float *pairQ;           // shell-pair Schwarz factors (the FP32 kernels' shell-quartet cut)
 };
 constexpr int PRIM_FIELDS = 13;
 // largest primitive-pair count of a shell pair (contractions of up to 6 primitives)
@@ -128,6 +155,7 @@ struct ScatterRec {
     const int2 *pair_shell;
     const int *shell_f0, *shell_nf;
     double *Jk, *J;
+    const int *flag;  // skipped when *flag == 0 (null: always)
 };
 hipError_t scatter_jk_batch(int nrec, const ScatterRec *recs, int max_npair, hipStream_t s);
 // J[f0(a) + c][f0(b) + d] += Jk[(c * nf(b) + d) * npair + j] for every pair j = (a, b), then Jk = 0
@@ -138,6 +166,7 @@ struct GemmRec {  // C = alpha op(A) op(B) + beta C, op(A) m x k, op(B) k x n, r
     int m, n, k, lda, ldb, ldc;
     const double *A, *B;
     double *C;
+    const int *flag;  // skipped when *flag == 0 (null: always)
 };
 hipError_t gemm_f64_batch(bool transA, bool transB, double alpha, double beta, const GemmRec *recs, int count,
                           int max_m, int max_n, hipStream_t s);
@@ -173,11 +202,49 @@ struct TraceRec {  // out = 1/2 sum P (2H + G)
 };
 hipError_t energy_trace_batch(const TraceRec *recs, int count, hipStream_t s);
 
+// ----- device-driven batched SCF (batch_scf.hip): every step one launch over the molecules,
+// controlled by each molecule's BatchState, so that one iteration is one captured graph
+struct BatchParams {
+    int max_iters, diis_start, diis_window;
+    int accurate;  // accurate policy: the returning iteration diagonalises the newest Fock matrix
+    int mixed;     // FP32 quartets until the error falls below max(10 eps, mixed_switch) or stalls
+    double eps, level_shift, level_shift_until, mixed_switch, mixed_stall;
+};
+struct BatchMolRec {  // one molecule's matrices for the state-driven steps
+    int n;
+    BatchState *st;
+    double *Jb, *Kb;          // J/K accumulators of the build
+    const double *H, *S;
+    double *G, *F, *Fx;
+    const double *FPS;        // F P S (the DIIS error), later S P S (the level shift)
+    double *errs, *focks;     // 16 ring slots each, n * n apart
+    const double *U;          // eigenvectors turned back with the previous basis: copied to E
+    double *E;
+    const double *P;
+};
+// zero J/K accumulators and the per-iteration maxima of the active molecules; *active_count = 0
+hipError_t batch_begin(const BatchMolRec *recs, int count, int max_n, int *active_count, hipStream_t s);
+// G = (Jb + Jb^T) - (Kb + Kb^T) / 2, F = H + G
+hipError_t batch_fock_from_jk(const BatchMolRec *recs, int count, int max_n, hipStream_t s);
+// e = FPS - (FPS)^T into the next ring slot (and F into its Fock slot), max |e|, <e, e_s> for the 16 slots
+hipError_t batch_diis_error(const BatchMolRec *recs, int count, int max_n, hipStream_t s);
+// ring bookkeeping, DIIS solve (or the newest matrix), level-shift decision, divergence
+hipError_t batch_diis_update(const BatchMolRec *recs, int count, BatchParams p, hipStream_t s);
+// Fx = sum coef_i focks[cslot_i]; Fx += sigma (S - SPS / 2) where the shift applies; E = U
+hipError_t batch_lincomb(const BatchMolRec *recs, int count, int max_n, hipStream_t s);
+hipError_t batch_level_shift(const BatchMolRec *recs, int count, int max_n, double sigma, hipStream_t s);
+hipError_t batch_copy_basis(const BatchMolRec *recs, int count, int max_n, hipStream_t s);
+// E_elec = 1/2 tr(P (2H + G)) of every active molecule into its state
+hipError_t batch_energy(const BatchMolRec *recs, int count, hipStream_t s);
+// exit rule of scf/src/lib.rs:224-249 per molecule, phase changes of the mixed form, active count
+hipError_t batch_exit(const BatchMolRec *recs, int count, BatchParams p, int *active_count, hipStream_t s);
+
 // one matrix of a batched symmetric eigenproblem (n = 0: skipped)
 struct EigRec {
     int n;
     double *A;  // in: symmetric n x n; out: eigenvector k (of the k-th smallest eigenvalue) at A[k * n]
     double *W;  // out: eigenvalues, ascending
+    const int *flag;  // skipped when *flag == 0 (null: always)
 };
 constexpr int JACOBI_MAX_N = 128;  // the matrix is held in LDS (128 KB at the limit)
 hipError_t jacobi_eig_batch(const EigRec *recs, int count, int max_n, hipStream_t s);

@@ -1,6 +1,7 @@
 """Batching form: many water clusters (H2O)_k, k uniform in 1..8, 6-31G, advanced in lockstep
 with one Fock-build launch per quartet class per iteration (fock_scf_batch_run). Each molecule is
-checked against the single-molecule path (fock_scf_run) through the same C ABI.
+checked against the single-molecule path (fock_scf_run) through the same C ABI and, for seeded
+(H2O)_2..4, against the oracle.
 """
 import numpy as np
 import pytest
@@ -88,8 +89,31 @@ def test_batch_matches_oracle_energies(ctx):
 
 def test_batch_rejects_mixed_precision(ctx):
     mols = clusters(2, seed=3)
-    with pytest.raises(abi.FockError):
+    with pytest.raises(abi.FockError) as e:
         ctx.scf_batch_run(mols, opts(precision=abi.PRECISION_MIXED))
+    assert e.value.status == abi.FOCK_UNSUPPORTED
+
+
+@pytest.mark.skip(reason="the device-driven batch iteration that runs the mixed form is switched off: "
+                         "its first GPU run ended in a GPU memory fault whose cause has not been found")
+def test_batch_mixed_precision_matches_oracle(ctx):
+    """Mixed form in the batch: FP32 quartet integrals until each molecule's error falls below
+    1e-4, FP64 builds from there (the result always comes from an FP64 build): the seeded (H2O)_2..4
+    against the oracle's energies, and incremental builds and the 1e-6 pair screen refused."""
+    import os
+    from conftest import REPO
+    d = np.load(os.path.join(REPO, "tests", "golden", "forms.npz"))
+    mols = [m.flatten(m.build(forms_geometry(d, "w%d" % k), "6-31G")) for k in (2, 3, 4)]
+    st, statuses, res = ctx.scf_batch_run(mols, opts(epsilon=1e-10, precision=abi.PRECISION_MIXED))
+    assert st == abi.FOCK_OK, statuses
+    for k, r in zip((2, 3, 4), res):
+        dE = abs(r["total_energy"] - d["w%d_energy" % k][0])
+        print("(H2O)%d mixed batch |dE| vs oracle %.2e, %d iterations" % (k, dE, r["iterations"]))
+        assert dE <= 1e-8
+    with pytest.raises(abi.FockError):
+        ctx.scf_batch_run(mols, opts(precision=abi.PRECISION_MIXED, incremental=1))
+    with pytest.raises(abi.FockError):
+        ctx.scf_batch_run(mols, opts(precision=abi.PRECISION_MIXED, screen=abi.SCREEN_REF_BF_PAIR))
 
 
 def test_python_facade_batch_and_gradient(ctx):
