@@ -394,6 +394,11 @@ def main():
                  "dominant_kernel": CLASS_NAMES[fp64_dom], "dominant_achieved_tflops": fp64_ach,
                  "dominant_frac": fp64_ach / PEAK_FP64},
         "speedup_mixed_vs_fp64": fp64["ms"] / mixed["ms"],
+        # per class and form: kept shell quartets, primitive quartets and integrals (the FP32
+        # kernels drop shell quartets below the FP32 cut), and the algorithmic flop behind roofline
+        "class_counts": {name: {CLASS_NAMES[c]: {k: int(per[name]["stats"][k][c]) for k in
+                                                 ("shell_quartets", "primitive_quartets", "integrals", "algorithmic_flop")}
+                                for c in range(len(CLASS_NAMES))} for name in ("mixed", "fp64")},
         "class_ms": {"mixed": dict(zip(CLASS_NAMES, map(float, mixed["class_ms"]))),
                      "fp64": dict(zip(CLASS_NAMES, map(float, fp64["class_ms"])))},
         "scf": scf, "e_oracle": e_oracle,

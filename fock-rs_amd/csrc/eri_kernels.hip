@@ -109,20 +109,42 @@ __device__ __forceinline__ PrimPair<Real> soa_prim(const Real *__restrict__ base
     return q;
 }
 
-// Same integrals with the bra primitive pairs (and their q) and the Boys slab in LDS, and the
-// ket primitive pairs read from the bucket-SoA copy (the 64 lanes of a task read consecutive
-// entries of every field): each ket primitive pair is loaded once per lane and reused for every
-// bra primitive. ket.pp_start is the pair's SoA start, ket_stride its bucket size.
-template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real>
-__device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const PairInfo &bra,
-                                                   const PrimPair<Real> *__restrict__ bra_pp,
-                                                   const float *__restrict__ bra_q, const PairInfo &ket,
-                                                   int ket_stride, bool active,
-                                                   const Real *slice,
+// Bra primitive pair i of a record read through the scalar data path (every lane of the workgroup
+// reads the same record).
+template <typename Real>
+__device__ __forceinline__ PrimPair<Real> scalar_prim(ConstPtr<Real> base, int i) {
+    const ConstPtr<Real> f = base + i * 12;
+    PrimPair<Real> b;
+    b.p = f[0];
+    b.Px = f[1];
+    b.Py = f[2];
+    b.Pz = f[3];
+    b.Kc = f[4];
+    b.xax = f[5];
+    b.xay = f[6];
+    b.xaz = f[7];
+    b.xbx = f[8];
+    b.xby = f[9];
+    b.xbz = f[10];
+    b.inv2p = f[11];
+    return b;
+}
+
+// Same integrals with the Boys slab in LDS, the ket primitive pairs read from the bucket-SoA copy
+// (the 64 lanes of a task read consecutive entries of every field: each ket primitive pair is
+// loaded once per lane and reused for every bra primitive), and the bra primitive pairs and their
+// q read with scalar loads (wave-uniform: SGPR operands, no LDS traffic). ket.pp_start is the
+// pair's SoA start, ket_stride its bucket size.
+template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool SLAB>
+__device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const PairInfo &bra, const PairInfo &ket,
+                                                   int ket_stride, bool active, LdsPtr<Real> slice,
                                                    Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)]) {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NAB = NA * NB, NCD = NC * ND;
     const Real *__restrict__ kbase = reinterpret_cast<const Real *>(args.soa) + ket.pp_start;
+    // the bra pair is the workgroup's: its range is uniform
+    const int bstart = __builtin_amdgcn_readfirstlane(bra.pp_start), bK = __builtin_amdgcn_readfirstlane(bra.K);
+    const ConstPtr<Real> cb = (ConstPtr<Real>)(reinterpret_cast<const Real *>(args.prims) + (size_t)bstart * 12);
 #pragma unroll
     for (int x = 0; x < NAB; ++x)
 #pragma unroll
@@ -130,6 +152,8 @@ __device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const Pai
     if (args.prim_q) {
         // primitive pairs are in descending q (bra and ket): the kept bra primitives of a ket
         // primitive form a prefix, and the wave stops at the first (k, i) no lane keeps
+        const ConstPtr<float> cq = (ConstPtr<float>)(args.prim_q + bstart);
+        const float q0 = cq[0], thr = args.prim_thr;
         if constexpr (sizeof(Real) == 4) {
             // FP32: ket primitive k + 1 is loaded while primitive k is evaluated (every pair has
             // K >= 1); in FP64 the extra registers cost the larger classes more than it saves
@@ -142,38 +166,32 @@ __device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const Pai
                     qn = active ? (float)kbase[((size_t)(k + 1) * PRIM_FIELDS + 12) * ket_stride] : 0.0f;
                     kn = soa_prim(kbase, k + 1, ket_stride);
                 }
-                if (__ballot(qk * bra_q[0] > args.prim_thr) == 0) break;
-                for (int i = 0; i < bra.K; ++i) {
-                    const bool keep = bra_q[i] * qk > args.prim_thr;
+                if (__ballot(qk * q0 > thr) == 0) break;
+                for (int i = 0; i < bK; ++i) {
+                    const bool keep = cq[i] * qk > thr;
                     if (__ballot(keep) == 0) break;
-                    if (keep) {
-                        const PrimPair<Real> b = bra_pp[i];
-                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
-                    }
+                    if (keep)
+                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(scalar_prim(cb, i), kq, args.bt, I, slice);
                 }
             }
         } else {
             for (int k = 0; k < ket.K; ++k) {
                 const float qk = active ? (float)kbase[((size_t)k * PRIM_FIELDS + 12) * ket_stride] : 0.0f;
-                if (__ballot(qk * bra_q[0] > args.prim_thr) == 0) break;
+                if (__ballot(qk * q0 > thr) == 0) break;
                 const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
-                for (int i = 0; i < bra.K; ++i) {
-                    const bool keep = bra_q[i] * qk > args.prim_thr;
+                for (int i = 0; i < bK; ++i) {
+                    const bool keep = cq[i] * qk > thr;
                     if (__ballot(keep) == 0) break;
-                    if (keep) {
-                        const PrimPair<Real> b = bra_pp[i];
-                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
-                    }
+                    if (keep)
+                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(scalar_prim(cb, i), kq, args.bt, I, slice);
                 }
             }
         }
     } else {
         for (int k = 0; k < ket.K; ++k) {
             const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
-            for (int i = 0; i < bra.K; ++i) {
-                const PrimPair<Real> b = bra_pp[i];
-                accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real>(b, kq, args.bt, I, slice);
-            }
+            for (int i = 0; i < bK; ++i)
+                accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(scalar_prim(cb, i), kq, args.bt, I, slice);
         }
     }
     const int n = args.nbf;
@@ -461,7 +479,6 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     // the slab is staged in LDS for the FP32 kernels and FP64 (ss|ss); the other FP64 classes read
     // the Boys table from global memory (their 33 KB slab cost more occupancy than the L2 reads)
     constexpr bool SLAB = POLICY == 1 && block_slab_in_lds(L, sizeof(Real) == 4);
-    static_assert(MAX_PRIM_PAIRS * (sizeof(PrimPair<Real>) + sizeof(float)) % 16 == 0, "LDS slab alignment");
     // LDS: bra primitive pairs | their q | Boys slab | K rows a, b | J_ab partials of the 4 waves |
     // the block's tasks | its kept ket pairs
     extern __shared__ __align__(32) unsigned char lds_bytes[];
@@ -497,10 +514,9 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     }
     const Block blk = blk_;
     const int n = args.nbf;  // LDS is sized for the largest basis of the launch
-    PrimPair<Real> *bra_pp = reinterpret_cast<PrimPair<Real> *>(lds_bytes);  // [MAX_PRIM_PAIRS]
-    float *bra_q = reinterpret_cast<float *>(bra_pp + MAX_PRIM_PAIRS);       // [MAX_PRIM_PAIRS]
-    Real *slice = reinterpret_cast<Real *>(bra_q + MAX_PRIM_PAIRS);          // [BOYS_GRID_N][COLS]
-    double *lds = reinterpret_cast<double *>(slice + (SLAB ? (GN * COLS + 1) / 2 * 2 : 0));
+    // the class's Boys slab: static LDS ahead of the dynamic part (32-bit LDS addressing)
+    __shared__ Real slice[SLAB ? GN * COLS : 1];  // [GN][COLS]
+    double *lds = reinterpret_cast<double *>(lds_bytes);
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
     double *red = Kb + NB * n;        // [4][NAB]
@@ -556,11 +572,6 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     }
     for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
     {
-        const PrimPair<Real> *gpp = reinterpret_cast<const PrimPair<Real> *>(args.prims) + bra.pp_start;
-        for (int e = threadIdx.x; e < bra.K; e += blockDim.x) {
-            bra_pp[e] = gpp[e];
-            bra_q[e] = args.prim_q ? args.prim_q[bra.pp_start + e] : 0.0f;
-        }
         if constexpr (SLAB) {
             const Real *slab;
             if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
@@ -591,8 +602,8 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         const PairInfo ket = args.soa_pairs[ket_idx];
         const int ket_stride = args.soa_stride[ket_idx];
         Real I[NAB][NCD];
-        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real>(args, bra, bra_pp, bra_q, ket, ket_stride, active,
-                                                                    SLAB ? slice : nullptr, I);
+        lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real, SLAB>(args, bra, ket, ket_stride, active,
+                                                                          (LdsPtr<Real>)slice, I);
         const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
         if (fa == fb) w *= Real(0.5);
@@ -718,16 +729,18 @@ static hipError_t dispatch_policy(int cls, const JKArgs &a, int policy, bool ref
 }
 
 // LDS of one workgroup of the blocked digestion: K rows a and b, the J_ab reduction, the tile's
-// task and kept-ket records, the bra primitive pairs with their q, and the class's Boys slab
-size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile) {
+// task and kept-ket records, and the class's Boys slab
+// (the dynamic part; the Boys slab is a static array of the kernel)
+static size_t block_lds_dynamic(int cls, int nbf, int tile) {
     const int na = CLASS_BRA[cls] == 0 ? 1 : 3, nb = CLASS_BRA[cls] == 2 ? 3 : 1;
+    return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
+           (size_t)tile * (sizeof(Task) + 64 * sizeof(int) + sizeof(unsigned long long));
+}
+size_t block_lds_bytes(int cls, int nbf, int policy, bool fp32, int tile) {
     const size_t cols = fp32 ? BOYS_PD_COLS32 : BOYS_PD_COLS64, real = fp32 ? sizeof(float) : sizeof(double);
     const size_t rows = fp32 ? BOYS_PD32_N : BOYS_GRID_N;
-    const size_t pp = fp32 ? sizeof(PrimPair<float>) : sizeof(PrimPair<double>);
-    return (size_t)(na + nb) * nbf * sizeof(double) + 4 * na * nb * sizeof(double) +
-           (size_t)tile * (sizeof(Task) + 64 * sizeof(int) + sizeof(unsigned long long)) +
-           MAX_PRIM_PAIRS * (pp + sizeof(float)) +
-           ((policy == 1 && block_slab_in_lds(CLASS_BRA[cls] + CLASS_KET[cls], fp32)) ? (rows * cols + 1) / 2 * 2 * real : 0);
+    const bool slab = policy == 1 && block_slab_in_lds(CLASS_BRA[cls] + CLASS_KET[cls], fp32);
+    return block_lds_dynamic(cls, nbf, tile) + (slab ? rows * cols * real : 0) + 32;  // + alignment of the dynamic part
 }
 
 template <int LA, int LB, int LC, int LD, int POLICY, bool REFSCREEN, typename Real, bool BATCH = false>
@@ -735,7 +748,7 @@ static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
     if (a.tile < 1 || a.tile > MAX_TILE) return hipErrorInvalidValue;
     constexpr int cls = (LA + LB == 0) ? 0 : (LA + LB == 1) ? (LC + LD == 0 ? 1 : 2) : (LC + LD == 0 ? 3 : LC + LD == 1 ? 4 : 5);
-    const size_t lds = block_lds_bytes(cls, a.nbf, POLICY, sizeof(Real) == 4, a.tile);
+    const size_t lds = block_lds_dynamic(cls, a.nbf, a.tile);
     hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real, BATCH>), dim3(a.nblocks), dim3(256),
                        lds, s, a);
     return hipGetLastError();

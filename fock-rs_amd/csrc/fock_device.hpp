@@ -110,6 +110,19 @@ struct BoysTables {
 
 __device__ __forceinline__ double rcp_exact(double x) { return 1.0 / x; }
 
+// Pointers into the constant (scalar-load) and LDS address spaces. A load through a ConstPtr at a
+// wave-uniform address is a scalar load whose value is an SGPR operand of the vector
+// instructions; indexing through an LdsPtr keeps the address arithmetic in 32 bits.
+template <typename Real>
+using ConstPtr = const Real __attribute__((address_space(4))) *;
+template <typename Real>
+using LdsPtr = const Real __attribute__((address_space(3))) *;
+
+// 1/sqrt(x) for the FP32 kernels, x a normal positive number (p + q, T >= BOYS_TMAX): the hardware
+// reciprocal square root (1 ulp) without the rescaling that the library adds for subnormal x
+__device__ __forceinline__ float rsqrt_normal(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_normal(double x) { return rsqrt(x); }
+
 // (2n-1)!!, n >= 0 (boys/src/lib.rs:5-10 with the saturating 2n-1 of :67)
 __host__ __device__ constexpr double dfact_odd(int n) {
     double p = 1.0;
@@ -249,15 +262,16 @@ __device__ __forceinline__ void boys_taylor32(float T, const float *__restrict__
 // F_0..F_L from the order-L slab of the pre-divided table (staged in LDS): Horner in FMAs for
 // F_L, exp(-T) = exp(-T_i) e^{T_i - T} with the 1/k! constants, then downward recursion.
 template <int L, typename Real>
-__device__ __forceinline__ void boys_taylor_lds(Real T, const Real *__restrict__ slab, Real F[L + 1]) {
+__device__ __forceinline__ void boys_taylor_lds(Real T, LdsPtr<Real> slab, Real F[L + 1]) {
     constexpr int TERMS = sizeof(Real) == 8 ? BOYS_TAYLOR64 : BOYS_PD32_TERMS;
     constexpr int COLS = TERMS + 1;
     if (T < Real(BOYS_TMAX)) {
+        // T = alpha |P - Q|^2 >= 0, so the rounded index is >= 0
         int i;
-        if constexpr (sizeof(Real) == 8) i = (int)fmax(T * 10.0 + 0.5, 0.0);
-        else i = (int)fmaxf(T * 5.0f + 0.5f, 0.0f);
+        if constexpr (sizeof(Real) == 8) i = (int)(T * 10.0 + 0.5);
+        else i = (int)(T * 5.0f + 0.5f);
         const Real d = (Real)i * Real(sizeof(Real) == 8 ? BOYS_GRID_STEP : BOYS_PD32_STEP) - T;
-        const Real *row = slab + i * COLS;
+        const LdsPtr<Real> row = slab + __umul24(i, COLS);  // i < 2^24: the full-rate 24-bit multiply
         Real acc = row[TERMS - 1];
 #pragma unroll
         for (int k = TERMS - 2; k >= 0; --k) acc = acc * d + row[k];
@@ -284,9 +298,7 @@ __device__ __forceinline__ void boys_taylor_lds(Real T, const Real *__restrict__
         }
     } else {
         static_assert(L <= 4, "extend the grid or keep exp(-T) for higher orders");
-        Real r;
-        if constexpr (sizeof(Real) == 8) r = rsqrt(T);
-        else r = rsqrtf(T);
+        const Real r = rsqrt_normal(T);
         F[0] = Real(0.886226925452758) * r;
         const Real h = Real(0.5) * r * r;
 #pragma unroll
@@ -369,36 +381,35 @@ __device__ __forceinline__ void hermite_coulomb(const Real Fs[L + 1], Real X, Re
 // Accumulate one primitive quartet into the contracted integrals I[ab][cd] (bra components
 // a-major, ket components c-major). POLICY: 0 = reference arithmetic (BoysRef + RBug),
 // 1 = accurate. The bra and ket primitive pairs are passed by value.
-template <int LA, int LB, int LC, int LD, int POLICY, typename Real>
+template <int LA, int LB, int LC, int LD, int POLICY, typename Real, bool SLAB = false>
 __device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &bra, const PrimPair<Real> &ket,
                                                         const BoysTables &bt,
                                                         Real I[ncart(LA) * ncart(LB)][ncart(LC) * ncart(LD)],
-                                                        const Real *lds_slice = nullptr) {
+                                                        LdsPtr<Real> lds_slice = nullptr) {
     constexpr int L = LA + LB + LC + LD;
     constexpr int LAB = LA + LB;
     constexpr int LCD = LC + LD;
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     const Real p = bra.p, q = ket.p;
     const Real pq = p + q;
-    Real rs, alpha;
-    if constexpr (sizeof(Real) == 8) {
-        rs = rsqrt(pq);
-    } else {
-        rs = rsqrtf(pq);
-    }
-    alpha = p * q * (rs * rs);
+    const Real rs = rsqrt_normal(pq);
+    const Real alpha = p * q * (rs * rs);
     // comp_diff = Q - P (contracted_gaussian.rs:294)
     const Real X = ket.Px - bra.Px, Y = ket.Py - bra.Py, Z = ket.Pz - bra.Pz;
     const Real T = alpha * (X * X + Y * Y + Z * Z);
-    // 2 pi^{5/2} / (p q sqrt(p+q)) with 1/p, 1/q inside Kc
-    const Real pref = Real(34.98683665524972) * bra.Kc * ket.Kc * rs;
+    // 2 pi^{5/2} / (p q sqrt(p+q)) with 1/p, 1/q inside Kc; the accurate policy groups the ket
+    // factor (the same for every bra primitive of the inner loop, so it is computed once per ket
+    // primitive)
+    Real pref;
+    if constexpr (POLICY == 0) pref = Real(34.98683665524972) * bra.Kc * ket.Kc * rs;
+    else pref = bra.Kc * (Real(34.98683665524972) * ket.Kc) * rs;
     Real Fs[L + 1];
     if constexpr (POLICY == 0) {
         double Fd[L + 1];
         boys_ref<L>((double)T, bt.lut_ref, Fd);
 #pragma unroll
         for (int n = 0; n <= L; ++n) Fs[n] = (Real)Fd[n];
-    } else if (lds_slice) {
+    } else if constexpr (SLAB) {
         boys_taylor_lds<L, Real>(T, lds_slice, Fs);
     } else if constexpr (sizeof(Real) == 8) {
         boys_taylor64<L>(T, bt.tab64, Fs);
@@ -449,7 +460,7 @@ __device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &br
                     constexpr int u = decltype(u_)::value;
                     static_for<0, LAB + 1 - t - u>([&](auto v_) {
                         constexpr int v = decltype(v_)::value;
-                        Real acc = Real(0);
+                        Real acc;  // the first term initialises it (no 0 + x)
                         static_for<0, cx + dx + 1>([&](auto tt_) {
                             constexpr int tt = decltype(tt_)::value;
                             static_for<0, cy + dy + 1>([&](auto uu_) {
@@ -457,7 +468,8 @@ __device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &br
                                 static_for<0, cz + dz + 1>([&](auto vv_) {
                                     constexpr int vv = decltype(vv_)::value;
                                     Real e = Ecd[0][cx][dx][tt] * Ecd[1][cy][dy][uu] * Ecd[2][cz][dz][vv];
-                                    if constexpr ((tt + uu + vv) & 1) acc -= e * R[t + tt][u + uu][v + vv];
+                                    if constexpr (tt + uu + vv == 0) acc = e * R[t][u][v];
+                                    else if constexpr ((tt + uu + vv) & 1) acc -= e * R[t + tt][u + uu][v + vv];
                                     else acc += e * R[t + tt][u + uu][v + vv];
                                 });
                             });
@@ -472,14 +484,16 @@ __device__ __forceinline__ void accumulate_prim_quartet(const PrimPair<Real> &br
                 static_for<0, NB>([&](auto b_) {
                     constexpr int b = decltype(b_)::value;
                     constexpr int bx = cart(LB, b, 0), by = cart(LB, b, 1), bz = cart(LB, b, 2);
-                    Real acc = Real(0);
+                    Real acc;  // the first term initialises it (no 0 + x)
                     static_for<0, ax + bx + 1>([&](auto t_) {
                         constexpr int t = decltype(t_)::value;
                         static_for<0, ay + by + 1>([&](auto u_) {
                             constexpr int u = decltype(u_)::value;
                             static_for<0, az + bz + 1>([&](auto v_) {
                                 constexpr int v = decltype(v_)::value;
-                                acc += Eab[0][ax][bx][t] * Eab[1][ay][by][u] * Eab[2][az][bz][v] * Xk[t][u][v];
+                                const Real term = Eab[0][ax][bx][t] * Eab[1][ay][by][u] * Eab[2][az][bz][v] * Xk[t][u][v];
+                                if constexpr (t + u + v == 0) acc = term;
+                                else acc += term;
                             });
                         });
                     });
