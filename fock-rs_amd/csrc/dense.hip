@@ -13,15 +13,18 @@ namespace fock {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // C = alpha op(A) op(B) + beta C on the FP64 matrix cores (v_mfma_f64_16x16x4): one workgroup per
-// 16 x 16 tile of C, the K range split over its four waves (wave w takes the k blocks 4w, 4w + 16,
+// 16 x 16 tile of C, the K range split over its NW waves (wave w takes the k blocks 4w, 4w + 4 NW,
 // ...), the partial tiles summed through LDS. Small tiles and the split keep the SCF's n <= 600
 // products spread over the CUs (n = 264: 289 workgroups) and each wave's chain of dependent
-// load -> MFMA steps a quarter as long.
-__device__ __forceinline__ void gemm_f64_tile(bool transA, bool transB, int M, int N, int K, double alpha,
-                                              const double *__restrict__ A, int lda, const double *__restrict__ B,
-                                              int ldb, double beta, double *__restrict__ C, int ldc, int row0,
-                                              int col0) {
-    __shared__ double4_t part[3][64];  // partial tiles of waves 1..3
+// load -> MFMA steps short: with eight waves and eight k blocks of operands requested before their
+// MFMAs, a wave of an n = 264 product waits for memory once.
+// The product tile op(A) op(B) of rows row0.., columns col0.. in wave 0's accumulators (element
+// [row0 + 4 r + lane / 16][col0 + lane % 16] in acc[r]); the other waves return zeros.
+template <int NW>
+__device__ __forceinline__ double4_t gemm_f64_tile_acc(bool transA, bool transB, int M, int N, int K,
+                                                       const double *__restrict__ A, int lda,
+                                                       const double *__restrict__ B, int ldb, int row0, int col0) {
+    __shared__ double4_t part[NW - 1][64];  // partial tiles of waves 1..NW-1
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int ar = row0 + li, bc = col0 + li;
@@ -32,20 +35,34 @@ __device__ __forceinline__ void gemm_f64_tile(bool transA, bool transB, int M, i
     auto load_b = [&](int kk) -> double {
         return (bc < N && kk < K) ? (transB ? B[(size_t)bc * ldb + kk] : B[(size_t)kk * ldb + bc]) : 0.0;
     };
-    // the operands of the next k block are loaded while the current one is multiplied
-    double a = load_a(4 * wave + lk), b = load_b(4 * wave + lk);
-    for (int k0 = 4 * wave; k0 < K; k0 += 16) {
-        const double an = load_a(k0 + 16 + lk), bn = load_b(k0 + 16 + lk);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-        a = an;
-        b = bn;
+    // DEPTH k blocks of operands are requested before their MFMAs (blocks past K load zeros)
+    constexpr int DEPTH = 8;
+    for (int k0 = 4 * wave; k0 < K; k0 += 4 * NW * DEPTH) {
+        double a[DEPTH], b[DEPTH];
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) {
+            a[u] = load_a(k0 + 4 * NW * u + lk);
+            b[u] = load_b(k0 + 4 * NW * u + lk);
+        }
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
     if (wave > 0) part[wave - 1][lane] = acc;
     __syncthreads();
-    if (wave != 0) return;
-    acc += part[0][lane];
-    acc += part[1][lane];
-    acc += part[2][lane];
+    if (wave != 0) return double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < NW - 1; ++w) acc += part[w][lane];
+    return acc;
+}
+
+template <int NW>
+__device__ __forceinline__ void gemm_f64_tile(bool transA, bool transB, int M, int N, int K, double alpha,
+                                              const double *__restrict__ A, int lda, const double *__restrict__ B,
+                                              int ldb, double beta, double *__restrict__ C, int ldc, int row0,
+                                              int col0) {
+    const double4_t acc = gemm_f64_tile_acc<NW>(transA, transB, M, N, K, A, lda, B, ldb, row0, col0);
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = row0 + 4 * r + lk, j = col0 + li;
@@ -62,11 +79,13 @@ __device__ __forceinline__ void gemm_f64_tile(bool transA, bool transB, int M, i
 // ...), the partial tiles summed through LDS. Small tiles and the split keep the SCF's n <= 600
 // products spread over the CUs (n = 264: 289 workgroups) and each wave's chain of dependent
 // load -> MFMA steps a quarter as long.
-__global__ __launch_bounds__(256) void gemm_f64_mfma_kernel(bool transA, bool transB, int M, int N, int K,
-                                                            double alpha, const double *__restrict__ A, int lda,
-                                                            const double *__restrict__ B, int ldb, double beta,
-                                                            double *__restrict__ C, int ldc) {
-    gemm_f64_tile(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.y * 16, blockIdx.x * 16);
+constexpr int GEMM_NW = 8;  // waves per tile of the single products
+__global__ __launch_bounds__(64 * GEMM_NW) void gemm_f64_mfma_kernel(bool transA, bool transB, int M, int N, int K,
+                                                                    double alpha, const double *__restrict__ A, int lda,
+                                                                    const double *__restrict__ B, int ldb, double beta,
+                                                                    double *__restrict__ C, int ldc) {
+    gemm_f64_tile<GEMM_NW>(transA, transB, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.y * 16,
+                           blockIdx.x * 16);
 }
 
 // the same over a batch: blockIdx.z is the record; tiles outside a record's C return at once
@@ -76,7 +95,7 @@ __global__ __launch_bounds__(256) void gemm_f64_batch_kernel(bool transA, bool t
     const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
     if (row0 >= r.m || col0 >= r.n) return;  // the same for the whole workgroup
     if (r.flag && *r.flag == 0) return;      // the same for the whole workgroup
-    gemm_f64_tile(transA, transB, r.m, r.n, r.k, alpha, r.A, r.lda, r.B, r.ldb, beta, r.C, r.ldc, row0, col0);
+    gemm_f64_tile<4>(transA, transB, r.m, r.n, r.k, alpha, r.A, r.lda, r.B, r.ldb, beta, r.C, r.ldc, row0, col0);
 }
 
 hipError_t gemm_f64_batch(bool transA, bool transB, double alpha, double beta, const GemmRec *recs, int count,
@@ -90,7 +109,7 @@ hipError_t gemm_f64_batch(bool transA, bool transB, double alpha, double beta, c
 hipError_t gemm_f64(bool transA, bool transB, int m, int n, int k, double alpha, const double *A, int lda,
                     const double *B, int ldb, double beta, double *C, int ldc, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
-    dim3 grid((n + 15) / 16, (m + 15) / 16), block(256);
+    dim3 grid((n + 15) / 16, (m + 15) / 16), block(64 * GEMM_NW);
     hipLaunchKernelGGL(gemm_f64_mfma_kernel, grid, block, 0, s, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta,
                        C, ldc);
     return hipGetLastError();
@@ -167,26 +186,50 @@ hipError_t antisym_maxabs(int n, const double *M, double *e, double *maxabs, hip
 }
 
 // one workgroup per dot product: deterministic reduction order
-__global__ __launch_bounds__(256) void frob_dots_kernel(int64_t len, const double *__restrict__ a,
-                                                        const double *__restrict__ base, int64_t stride,
-                                                        double *__restrict__ dots) {
-    __shared__ double red[256];
-    const double *b = base + (int64_t)blockIdx.x * stride;
-    double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < len; i += blockDim.x) acc += a[i] * b[i];
-    red[threadIdx.x] = acc;
+// Sum over the workgroup of NT threads (a power of two) in a fixed order; thread 0 holds it.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[NT];
+    red[threadIdx.x] = v;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+#pragma unroll
+    for (int off = NT / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) dots[blockIdx.x] = red[0];
+    return red[0];
+}
+
+// sum_i f(i) over i < len by NT threads, four independent loads per thread in flight (the SCF's
+// reductions over n^2 elements are latency-bound otherwise); fixed order
+template <int NT, typename Fn>
+__device__ __forceinline__ double block_reduce(int64_t len, Fn f) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * NT < len; i += 4 * NT) {
+        a0 += f(i);
+        a1 += f(i + NT);
+        a2 += f(i + 2 * NT);
+        a3 += f(i + 3 * NT);
+    }
+    for (; i < len; i += NT) a0 += f(i);
+    return block_sum<NT>((a0 + a1) + (a2 + a3));
+}
+
+constexpr int REDUCE_NT = 1024;
+
+__global__ __launch_bounds__(REDUCE_NT) void frob_dots_kernel(int64_t len, const double *__restrict__ a,
+                                                              const double *__restrict__ base, int64_t stride,
+                                                              double *__restrict__ dots) {
+    const double *b = base + (int64_t)blockIdx.x * stride;
+    const double v = block_reduce<REDUCE_NT>(len, [&](int64_t i) { return a[i] * b[i]; });
+    if (threadIdx.x == 0) dots[blockIdx.x] = v;
 }
 
 hipError_t frob_dots(int64_t len, const double *a, const double *base, int64_t stride, int count, double *dots,
                      hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(frob_dots_kernel, dim3(count), dim3(256), 0, s, len, a, base, stride, dots);
+    hipLaunchKernelGGL(frob_dots_kernel, dim3(count), dim3(REDUCE_NT), 0, s, len, a, base, stride, dots);
     return hipGetLastError();
 }
 
@@ -213,29 +256,22 @@ hipError_t density_from_coeffs(int n, int nocc, const double *C, double *P, hipS
     return gemm_f64(false, true, n, n, nocc, 2.0, C, n, C, n, 0.0, P, n, s);
 }
 
+template <int NT>
 __device__ __forceinline__ void energy_trace_kernel_body(int n, const double *__restrict__ P,
                                                          const double *__restrict__ H, const double *__restrict__ G,
                                                          double *__restrict__ out) {
-    __shared__ double red[256];
-    const int64_t len = (int64_t)n * n;
-    double acc = 0.0;
-    for (int64_t e = threadIdx.x; e < len; e += blockDim.x) acc += P[e] * (2.0 * H[e] + G[e]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = 0.5 * red[0];
+    const double v = block_reduce<NT>((int64_t)n * n, [&](int64_t e) { return P[e] * (2.0 * H[e] + G[e]); });
+    if (threadIdx.x == 0) out[0] = 0.5 * v;
 }
-__global__ __launch_bounds__(256) void energy_trace_kernel(int n, const double *__restrict__ P,
-                                                           const double *__restrict__ H,
-                                                           const double *__restrict__ G, double *__restrict__ out) {
-    energy_trace_kernel_body(n, P, H, G, out);
+__global__ __launch_bounds__(REDUCE_NT) void energy_trace_kernel(int n, const double *__restrict__ P,
+                                                                 const double *__restrict__ H,
+                                                                 const double *__restrict__ G,
+                                                                 double *__restrict__ out) {
+    energy_trace_kernel_body<REDUCE_NT>(n, P, H, G, out);
 }
 
 hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(energy_trace_kernel, dim3(1), dim3(256), 0, s, n, P, H, G, out);
+    hipLaunchKernelGGL(energy_trace_kernel, dim3(1), dim3(REDUCE_NT), 0, s, n, P, H, G, out);
     return hipGetLastError();
 }
 
@@ -378,93 +414,209 @@ __global__ void scatter_jk_kernel(int npair, const int2 *__restrict__ pair_shell
 }
 // Gershgorin bounds of the spectrum of a symmetric matrix: out = {min_i (A_ii - r_i), max_i (A_ii + r_i)},
 // r_i = sum_{j != i} |A_ij|; one workgroup.
-__global__ __launch_bounds__(256) void gershgorin_kernel(int n, const double *__restrict__ A, double *__restrict__ out) {
-    __shared__ double lo[256], hi[256];
+// (one wave per row: the 64 lanes read the row coalesced; 16 waves)
+__global__ __launch_bounds__(1024) void gershgorin_kernel(int n, const double *__restrict__ A, double *__restrict__ out) {
+    __shared__ double lo[16], hi[16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double l = HUGE_VAL, h = -HUGE_VAL;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = wave; i < n; i += 16) {
+        const double *row = A + (size_t)i * n;
         double r = 0.0;
-        for (int j = 0; j < n; ++j)
-            if (j != i) r += fabs(A[(size_t)i * n + j]);
-        const double d = A[(size_t)i * n + i];
+        for (int j = lane; j < n; j += 64)
+            if (j != i) r += fabs(row[j]);
+        r = wave_sum(r);
+        const double d = row[i];
         l = fmin(l, d - r);
         h = fmax(h, d + r);
     }
-    lo[threadIdx.x] = l;
-    hi[threadIdx.x] = h;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
-            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
-        }
-        __syncthreads();
+    if (lane == 0) {
+        lo[wave] = l;
+        hi[wave] = h;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        out[0] = lo[0];
-        out[1] = hi[0];
+        double a = lo[0], b = hi[0];
+        for (int w = 1; w < 16; ++w) {
+            a = fmin(a, lo[w]);
+            b = fmax(b, hi[w]);
+        }
+        out[0] = a;
+        out[1] = b;
     }
 }
 hipError_t gershgorin_bounds(int n, const double *A, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(gershgorin_kernel, dim3(1), dim3(256), 0, s, n, A, out);
+    hipLaunchKernelGGL(gershgorin_kernel, dim3(1), dim3(1024), 0, s, n, A, out);
     return hipGetLastError();
 }
 
-// X = (hi I - A) / (hi - lo) with {lo, hi} = bounds (device)
-__global__ void tc2_init_kernel(int n, const double *__restrict__ A, const double *__restrict__ bounds,
-                                double *__restrict__ X) {
+// Coupled Newton-Schulz iteration for S^{-1/2} (Higham, Functions of Matrices, 2008, sec. 6.3):
+// Y0 = S / c, Z0 = I; T = (3 I - Z Y) / 2, Y <- Y T, Z <- T Z; then Z -> (S / c)^{-1/2}.
+// Y0 = S / bound[1] (a Gershgorin upper bound of the spectrum) and Z0 = I
+__global__ void ns_init_kernel(int n, const double *__restrict__ S, const double *__restrict__ bound,
+                               double *__restrict__ Y, double *__restrict__ Z) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * n) return;
     const int i = (int)(e / n), j = (int)(e % n);
-    const double lo = bounds[0], hi = bounds[1];
-    X[e] = ((i == j ? hi : 0.0) - A[e]) / (hi - lo);
+    Y[e] = S[e] / bound[1];
+    Z[e] = i == j ? 1.0 : 0.0;
 }
-hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, hipStream_t s) {
+hipError_t ns_init(int n, const double *S, const double *bound, double *Y, double *Z, hipStream_t s) {
     const int64_t len = (int64_t)n * n;
-    hipLaunchKernelGGL(tc2_init_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, A, bounds, X);
+    hipLaunchKernelGGL(ns_init_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, S, bound, Y, Z);
     return hipGetLastError();
 }
-
-// out = {tr X, tr X2, grow}; grow = 1 when tr(2X - X2) is closer to target than tr X2; one workgroup
-__global__ __launch_bounds__(256) void trace2_kernel(int n, const double *__restrict__ X, const double *__restrict__ X2,
-                                                     double target, double *__restrict__ out) {
-    __shared__ double a[256], b[256];
-    double ta = 0.0, tb = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        ta += X[(size_t)i * n + i];
-        tb += X2[(size_t)i * n + i];
+// T = (3 I - ZY) / 2 and *res = max(*res, max |I - ZY|) (res starts at 0)
+__global__ __launch_bounds__(256) void ns_update_kernel(int n, const double *__restrict__ ZY, double *__restrict__ T,
+                                                        double *__restrict__ res) {
+    __shared__ double red[256];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double r = 0.0;
+    if (e < (int64_t)n * n) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        const double d = (i == j ? 1.0 : 0.0) - ZY[e];
+        T[e] = 0.5 * ((i == j ? 2.0 : 0.0) + d);
+        r = fabs(d);
+        if (d != d) r = d;
     }
-    a[threadIdx.x] = ta;
-    b[threadIdx.x] = tb;
+    red[threadIdx.x] = r;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            a[threadIdx.x] += a[threadIdx.x + s];
-            b[threadIdx.x] += b[threadIdx.x + s];
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const double o = red[threadIdx.x + off], m = red[threadIdx.x];
+            red[threadIdx.x] = (o != o || o > m) ? o : m;
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = a[0];
-        out[1] = b[0];
-        out[2] = fabs(2.0 * a[0] - b[0] - target) < fabs(b[0] - target) ? 1.0 : 0.0;
-    }
+    if (threadIdx.x == 0) atomic_max_nonneg(res, red[0]);
 }
-hipError_t trace2(int n, const double *X, const double *X2, double target, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(trace2_kernel, dim3(1), dim3(256), 0, s, n, X, X2, target, out);
+hipError_t ns_update(int n, const double *ZY, double *T, double *res, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(ns_update_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, ZY, T, res);
+    return hipGetLastError();
+}
+// X = (Z + Z^T) / (2 sqrt(bound[1]))
+__global__ void ns_finish_kernel(int n, const double *__restrict__ Z, const double *__restrict__ bound,
+                                 double *__restrict__ X) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    X[e] = (Z[e] + Z[(int64_t)j * n + i]) * (0.5 / sqrt(bound[1]));
+}
+hipError_t ns_finish(int n, const double *Z, const double *bound, double *X, hipStream_t s) {
+    const int64_t len = (int64_t)n * n;
+    hipLaunchKernelGGL(ns_finish_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, Z, bound, X);
     return hipGetLastError();
 }
 
-// X = 2 X - X2 (grow != 0) or X = X2, with the choice read from the device
-__global__ void tc2_update_kernel(int64_t len, const double *__restrict__ grow, double *__restrict__ X,
-                                  const double *__restrict__ X2) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
-    X[e] = *grow != 0.0 ? 2.0 * X[e] - X2[e] : X2[e];
+// sum over the workgroup of NT threads of (t, f), returned to every thread (fixed order)
+template <int NT>
+__device__ __forceinline__ void block_sum2(double &t, double &f) {
+    __shared__ double rt[NT], rf[NT];
+    rt[threadIdx.x] = t;
+    rf[threadIdx.x] = f;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rt[threadIdx.x] += rt[threadIdx.x + s];
+            rf[threadIdx.x] += rf[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    t = rt[0];
+    f = rf[0];
 }
-hipError_t tc2_update(int64_t len, const double *grow, double *X, const double *X2, hipStream_t s) {
-    hipLaunchKernelGGL(tc2_update_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, len, grow, X, X2);
+
+// X = (hi I - A) / (hi - lo) with {lo, hi} = bounds (device); part[block] = (tr, sum of squares)
+// of the block's elements
+__global__ __launch_bounds__(256) void tc2_init_kernel(int n, const double *__restrict__ A,
+                                                       const double *__restrict__ bounds, double *__restrict__ X,
+                                                       double *__restrict__ part) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 0.0, f = 0.0;
+    if (e < (int64_t)n * n) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        const double lo = bounds[0], hi = bounds[1];
+        const double v = ((i == j ? hi : 0.0) - A[e]) / (hi - lo);
+        X[e] = v;
+        t = i == j ? v : 0.0;
+        f = v * v;
+    }
+    block_sum2<256>(t, f);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = t;
+        part[2 * blockIdx.x + 1] = f;
+    }
+}
+int tc2_init_parts(int n) { return (int)(((int64_t)n * n + 255) / 256); }
+hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, double *part, hipStream_t s) {
+    hipLaunchKernelGGL(tc2_init_kernel, dim3((unsigned)tc2_init_parts(n)), dim3(256), 0, s, n, A, bounds, X, part);
     return hipGetLastError();
 }
+
+// One trace-correcting purification step in one launch: the traces of X come from the previous
+// launch's per-workgroup partials (tr X, and tr X^2 = sum of squares for symmetric X), every
+// workgroup takes the same choice (X^2, or 2X - X^2 when its trace is closer to nocc), multiplies
+// its 16 x 16 tile of X X on the matrix cores, writes the chosen update into Xout and its own
+// partials of the new matrix. hist (workgroup (0, 0)) = {tr X, tr X^2, grow} of the input.
+__global__ __launch_bounds__(64 * GEMM_NW) void tc2_step_kernel(int n, const double *__restrict__ X,
+                                                                double *__restrict__ Xout,
+                                                                const double *__restrict__ part_in, int npart_in,
+                                                                double *__restrict__ part_out, double nocc,
+                                                                double *__restrict__ hist) {
+    __shared__ int s_grow;
+    // the partials are requested before the product's operands (one memory wait for both); the
+    // choice is needed only by the update
+    double t = 0.0, f = 0.0;
+    for (int p = threadIdx.x; p < npart_in; p += blockDim.x) {
+        t += part_in[2 * p];
+        f += part_in[2 * p + 1];
+    }
+    const int row0 = blockIdx.y * 16, col0 = blockIdx.x * 16;
+    const double4_t acc = gemm_f64_tile_acc<GEMM_NW>(false, false, n, n, n, X, n, X, n, row0, col0);
+    block_sum2<64 * GEMM_NW>(t, f);
+    if (threadIdx.x == 0) {
+        const int grow = fabs(2.0 * t - f - nocc) < fabs(f - nocc) ? 1 : 0;
+        s_grow = grow;
+        if (blockIdx.x == 0 && blockIdx.y == 0) {
+            hist[0] = t;
+            hist[1] = f;
+            hist[2] = grow;
+        }
+    }
+    __syncthreads();
+    const bool grow = s_grow != 0;
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    double tt = 0.0, ff = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = row0 + 4 * r + lk, j = col0 + li;
+        if (i < n && j < n) {
+            const double x2 = acc[r];
+            const double v = grow ? 2.0 * X[(size_t)i * n + j] - x2 : x2;
+            Xout[(size_t)i * n + j] = v;
+            if (i == j) tt += v;
+            ff += v * v;
+        }
+    }
+    tt = wave_sum(tt);
+    ff = wave_sum(ff);
+    if (lane == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        part_out[2 * b] = tt;
+        part_out[2 * b + 1] = ff;
+    }
+}
+int tc2_step_parts(int n) { return ((n + 15) / 16) * ((n + 15) / 16); }
+hipError_t tc2_step(int n, const double *X, double *Xout, const double *part_in, int npart_in, double *part_out,
+                    double nocc, double *hist, hipStream_t s) {
+    const int nt = (n + 15) / 16;
+    hipLaunchKernelGGL(tc2_step_kernel, dim3(nt, nt), dim3(64 * GEMM_NW), 0, s, n, X, Xout, part_in, npart_in,
+                       part_out, nocc, hist);
+    return hipGetLastError();
+}
+
 
 // Batched symmetric eigensolver, one workgroup per matrix, the matrix in LDS. One-sided (Hestenes)
 // Jacobi on B = A + c I, with c from the Gershgorin bound so that every eigenvalue of B is at
@@ -709,7 +861,7 @@ hipError_t level_shift_batch(const ShiftRec *recs, int count, int64_t max_len, d
 
 __global__ __launch_bounds__(256) void energy_trace_batch_kernel(const TraceRec *__restrict__ recs) {
     const TraceRec r = recs[blockIdx.x];
-    energy_trace_kernel_body(r.n, r.P, r.H, r.G, r.out);
+    energy_trace_kernel_body<256>(r.n, r.P, r.H, r.G, r.out);
 }
 hipError_t energy_trace_batch(const TraceRec *recs, int count, hipStream_t s) {
     if (count <= 0) return hipSuccess;

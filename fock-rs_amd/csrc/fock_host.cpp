@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -130,13 +132,22 @@ struct Prepared {
     // last SCF prepared reuses the pair and task lists
     std::vector<double> key_d;
     std::vector<int32_t> key_i;
-    uint64_t pair_hash = 0;  // FNV-1a of the sorted pair records and primitive-pair data
+    uint64_t pair_hash = 0;  // FNV-1a of the sorted pair records and the primitive pairs' q
 };
 
 // FNV-1a over bytes, continuing from h.
+// FNV-1a over 64-bit words (the tail byte-wise): a run-to-run fingerprint of the screen lists,
+// cheap enough to take on every preparation (~10 MB of primitive data for C20H42/6-31G)
 uint64_t fnv1a(uint64_t h, const void *data, size_t bytes) {
     const unsigned char *p = static_cast<const unsigned char *>(data);
-    for (size_t i = 0; i < bytes; ++i) {
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h ^= w;
+        h *= 1099511628211ull;
+    }
+    for (; i < bytes; ++i) {
         h ^= p[i];
         h *= 1099511628211ull;
     }
@@ -524,26 +535,94 @@ extern "C" fock_status fock_jk_prepare(fock_ctx *ctx, const fock_basis_desc *bas
 }
 
 namespace {
-// f(lo, hi) over [0, n) in up to 8 host threads (pure host work: no HIP calls inside f); small
-// ranges run on the calling thread.
+// Host worker pool for the preparation loops: up to 7 threads started once per process (a thread
+// start costs ~50 us, and a C20H42 preparation runs a dozen parallel loops), sleeping on a
+// condition variable between jobs.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool pool;
+        return pool;
+    }
+    int workers() const { return (int)th_.size(); }
+    // run job(t) for t = 1..count-1 on the workers and job(0) on the caller; returns when all are done
+    template <typename J>
+    void run(int count, J &&job) {
+        std::function<void(int)> fn(std::forward<J>(job));
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            fn_ = &fn;
+            next_ = 1;
+            count_ = count;
+            pending_ = count - 1;
+            ++epoch_;
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+
+  private:
+    HostPool() {
+        const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+        for (int t = 0; t < std::min(7, hw - 1); ++t) th_.emplace_back([this] { loop(); });
+    }
+    void loop();
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::function<void(int)> *fn_ = nullptr;
+    int next_ = 0, count_ = 0, pending_ = 0;
+    uint64_t epoch_ = 0;
+    bool stop_ = false;
+};
 thread_local bool tl_serial = false;  // set in worker threads: nested parallel loops run serially
+void HostPool::loop() {
+    tl_serial = true;
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(m_);
+    for (;;) {
+        cv_.wait(lk, [&] { return stop_ || (epoch_ != seen && next_ < count_); });
+        if (stop_) return;
+        while (next_ < count_) {
+            const int t = next_++;
+            std::function<void(int)> *fn = fn_;
+            lk.unlock();
+            (*fn)(t);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+        seen = epoch_;
+    }
+}
+std::mutex pool_user;  // one parallel loop at a time on the pool (other callers run serially)
+
+// f(lo, hi) over [0, n) on the caller and the host pool (pure host work: no HIP calls inside f);
+// small ranges, nested loops and concurrent callers run on the calling thread.
 template <typename F>
 void parallel_for(int n, int grain, F f) {
     if (tl_serial) {
         f(0, n);
         return;
     }
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    const int T = std::min({8, hw, n / std::max(grain, 1)});
-    if (T <= 1) {
+    HostPool &pool = HostPool::get();
+    const int T = std::min({pool.workers() + 1, 8, n / std::max(grain, 1)});
+    std::unique_lock<std::mutex> use(pool_user, std::try_to_lock);
+    if (T <= 1 || !use.owns_lock()) {
         f(0, n);
         return;
     }
-    std::vector<std::thread> th;
-    th.reserve(T - 1);
-    for (int t = 1; t < T; ++t) th.emplace_back([&, t]() { f((int)((int64_t)n * t / T), (int)((int64_t)n * (t + 1) / T)); });
-    f(0, (int)((int64_t)n / T));
-    for (auto &x : th) x.join();
+    pool.run(T, [&](int t) { f((int)((int64_t)n * t / T), (int)((int64_t)n * (t + 1) / T)); });
 }
 
 // Blocks of class cls at a tile: runs of up to `tile` consecutive tasks of one bra pair (task
@@ -788,8 +867,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     }
     HIPCHK(hipMemcpyAsync(pr.d_pairs.p, info.data(), npair * sizeof(PairInfo), hipMemcpyHostToDevice, S(ctx)));
     // checksum of the screen lists' inputs: pair order and the (q-sorted) primitive-pair data
-    pr.pair_hash = fnv1a(fnv1a(1469598103934665603ull, info.data(), info.size() * sizeof(PairInfo)), prims.data(),
-                         prims.size() * sizeof(PrimPair<double>));
+    pr.pair_hash = fnv1a(fnv1a(1469598103934665603ull, info.data(), info.size() * sizeof(PairInfo)),
+                         prim_q_host.data(), prim_q_host.size() * sizeof(float));
 
     // buckets: contiguous runs of equal (class, K)
     struct Bucket {
@@ -812,10 +891,22 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         std::vector<int> stride(npair);
         const std::vector<float> &qh = prim_q_host;
         const std::vector<PrimPair<double>> &ph = prims;
-        size_t base = 0;
-        for (const Bucket &bk : buckets) {
-            const int nb = bk.end - bk.begin;
-            for (int j = bk.begin; j < bk.end; ++j) {
+        std::vector<size_t> bucket_base(buckets.size());
+        std::vector<int> bucket_of(npair);
+        {
+            size_t base = 0;
+            for (size_t bi = 0; bi < buckets.size(); ++bi) {
+                const Bucket &bk = buckets[bi];
+                bucket_base[bi] = base;
+                for (int j = bk.begin; j < bk.end; ++j) bucket_of[j] = (int)bi;
+                base += (size_t)(bk.end - bk.begin) * bk.K * PRIM_FIELDS;
+            }
+        }
+        parallel_for(npair, 256, [&](int lo, int hi) {  // every pair writes its own column of its bucket
+            for (int j = lo; j < hi; ++j) {
+                const Bucket &bk = buckets[bucket_of[j]];
+                const size_t base = bucket_base[bucket_of[j]];
+                const int nb = bk.end - bk.begin;
                 const PairHost &pj = pr.pairs[j];
                 sinfo[j] = PairInfo{(int)(base + (j - bk.begin)), pj.K, pj.fa, pj.fb};
                 stride[j] = nb;
@@ -827,13 +918,15 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
                         soa64[base + ((size_t)k * PRIM_FIELDS + x) * nb + (j - bk.begin)] = f[x];
                 }
             }
-            base += (size_t)nb * bk.K * PRIM_FIELDS;
-        }
+        });
         HIPCHK(pr.d_soa64.ensure(std::max<size_t>(soa64.size(), 1) * sizeof(double)));
         HIPCHK(hipMemcpyAsync(pr.d_soa64.p, soa64.data(), soa64.size() * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
         if (precision == 1) {
             std::vector<float> soa32(soa64.size());
-            for (size_t e = 0; e < soa64.size(); ++e) soa32[e] = (float)soa64[e];
+            parallel_for((int)((soa64.size() + 4095) / 4096), 4, [&](int lo, int hi) {
+                const size_t e1 = std::min(soa64.size(), (size_t)hi * 4096);
+                for (size_t e = (size_t)lo * 4096; e < e1; ++e) soa32[e] = (float)soa64[e];
+            });
             HIPCHK(pr.d_soa32.ensure(std::max<size_t>(soa32.size(), 1) * sizeof(float)));
             HIPCHK(hipMemcpyAsync(pr.d_soa32.p, soa32.data(), soa32.size() * sizeof(float), hipMemcpyHostToDevice,
                                   S(ctx)));
@@ -857,20 +950,38 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     st.n_basis = n;
     st.n_shells = ns;
     st.n_shell_pairs = npair;
-    // the task lists of the six classes are independent: built in parallel, joined in class order
-    struct ClassTasks {
+    // the task lists: pieces of (class, range of bra pairs) built in parallel (pairs are sorted by
+    // class, so a class's bras are one index range), joined in class and bra order
+    struct Piece {
+        int cls, lo, hi;  // bra pairs [lo, hi) of the class's bra pair class
         std::vector<Task> ct;
-        std::vector<int> gf, gl;
+        std::vector<int> cgid;  // (bra, bucket) group of every task, numbered within the piece
+        int ngroup = 0;
         int64_t quartets = 0, prim_quartets = 0;
-    } per[NUM_CLASSES];
-    parallel_for(NUM_CLASSES, 1, [&](int clo, int chi) {
-        for (int cls = clo; cls < chi; ++cls) {
-            ClassTasks &out = per[cls];
-            const int X = CLASS_BRA[cls], Y = CLASS_KET[cls];
-            std::vector<Task> &ct = out.ct;
-            std::vector<int> cgid;
-            int ngroup = 0;
-            for (int i = 0; i < npair; ++i) {
+    };
+    std::vector<Piece> pieces;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+        const int X = CLASS_BRA[cls];
+        int xlo = npair, xhi = 0;
+        for (int i = 0; i < npair; ++i)
+            if (pr.pairs[i].cls == X) {
+                xlo = std::min(xlo, i);
+                xhi = i + 1;
+            }
+        constexpr int PIECES = 16;
+        for (int t = 0; t < PIECES && xlo < xhi; ++t) {
+            Piece pc;
+            pc.cls = cls;
+            pc.lo = xlo + (int)((int64_t)(xhi - xlo) * t / PIECES);
+            pc.hi = xlo + (int)((int64_t)(xhi - xlo) * (t + 1) / PIECES);
+            if (pc.hi > pc.lo) pieces.push_back(std::move(pc));
+        }
+    }
+    parallel_for((int)pieces.size(), 1, [&](int plo, int phi) {
+        for (int pi = plo; pi < phi; ++pi) {
+            Piece &out = pieces[pi];
+            const int X = CLASS_BRA[out.cls], Y = CLASS_KET[out.cls];
+            for (int i = out.lo; i < out.hi; ++i) {
                 if (pr.pairs[i].cls != X) continue;
                 const double di = pr.pairs[i].dmax;
                 for (const Bucket &bk : buckets) {
@@ -902,42 +1013,67 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
                     }
                     for (int s = lo; s < hi; s += 64) {
                         const int cnt = std::min(64, hi - s);
-                        ct.push_back(Task{i, s, cnt, pr.pairs[i].K * bk.K});
-                        cgid.push_back(ngroup);
+                        out.ct.push_back(Task{i, s, cnt, pr.pairs[i].K * bk.K});
+                        out.cgid.push_back(out.ngroup);
                         out.quartets += cnt;
                         out.prim_quartets += (int64_t)cnt * pr.pairs[i].K * bk.K;
                     }
-                    if (hi > lo) ++ngroup;
+                    if (hi > lo) ++out.ngroup;
                 }
             }
-            // tasks stay bra-major (the tasks of one bra pair, and of one (bra, bucket) group, are
-            // adjacent); blocks of up to a tile of tasks of one bra pair feed the blocked digestion
-            out.gf.resize(ct.size());
-            out.gl.resize(ct.size());
-            for (size_t q = 0; q < ct.size(); ++q)
-                out.gf[q] = (q > 0 && cgid[q] == cgid[q - 1]) ? out.gf[q - 1] : (int)q;
-            for (size_t q = ct.size(); q-- > 0;)
-                out.gl[q] = (q + 1 < ct.size() && cgid[q] == cgid[q + 1]) ? out.gl[q + 1] : (int)q;
         }
     });
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
-        const ClassTasks &c = per[cls];
-        pr.task_off[cls] = (int)tasks.size();
-        gfirst_all.insert(gfirst_all.end(), c.gf.begin(), c.gf.end());
-        glast_all.insert(glast_all.end(), c.gl.begin(), c.gl.end());
-        pr.max_slots = std::max<int64_t>(pr.max_slots, (int64_t)c.ct.size() * 64);
-        tasks.insert(tasks.end(), c.ct.begin(), c.ct.end());
-        st.shell_quartets[cls] = c.quartets;
-        st.primitive_quartets[cls] = c.prim_quartets;
-        st.tasks[cls] = (int64_t)c.ct.size();
-        st.integrals[cls] = st.shell_quartets[cls] * CLASS_NINT[cls];
-        st.algorithmic_flop[cls] = (double)st.primitive_quartets[cls] * F_PRIM[cls] + 12.0 * (double)st.integrals[cls];
+    mark("task_pieces");
+    // join: every piece's tasks and group ranges go straight to their final position (class order,
+    // then bra order); a (bra, bucket) group never spans two pieces
+    std::vector<size_t> piece_at(pieces.size());  // global task index of the piece's first task
+    std::vector<int> piece_local(pieces.size());  // its index within the class
+    {
+        size_t at = 0;
+        for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+            pr.task_off[cls] = (int)at;
+            int local = 0;
+            int64_t quartets = 0, prim_quartets = 0;
+            for (size_t pi = 0; pi < pieces.size(); ++pi) {
+                if (pieces[pi].cls != cls) continue;
+                piece_at[pi] = at;
+                piece_local[pi] = local;
+                at += pieces[pi].ct.size();
+                local += (int)pieces[pi].ct.size();
+                quartets += pieces[pi].quartets;
+                prim_quartets += pieces[pi].prim_quartets;
+            }
+            pr.max_slots = std::max<int64_t>(pr.max_slots, (int64_t)local * 64);
+            st.shell_quartets[cls] = quartets;
+            st.primitive_quartets[cls] = prim_quartets;
+            st.tasks[cls] = local;
+            st.integrals[cls] = st.shell_quartets[cls] * CLASS_NINT[cls];
+            st.algorithmic_flop[cls] =
+                (double)st.primitive_quartets[cls] * F_PRIM[cls] + 12.0 * (double)st.integrals[cls];
+        }
+        pr.task_off[NUM_CLASSES] = (int)at;
+        tasks.resize(at);
+        gfirst_all.resize(at);
+        glast_all.resize(at);
     }
-    pr.task_off[NUM_CLASSES] = (int)tasks.size();
-    pr.h_tasks = tasks;
+    parallel_for((int)pieces.size(), 1, [&](int plo, int phi) {
+        for (int pi = plo; pi < phi; ++pi) {
+            const Piece &pc = pieces[pi];
+            const size_t at = piece_at[pi];
+            const int local = piece_local[pi], m = (int)pc.ct.size();
+            std::copy(pc.ct.begin(), pc.ct.end(), tasks.begin() + at);
+            // class-local first and last task of every task's (bra, bucket) group
+            for (int q = 0; q < m; ++q)
+                gfirst_all[at + q] = (q > 0 && pc.cgid[q] == pc.cgid[q - 1]) ? gfirst_all[at + q - 1] : local + q;
+            for (int q = m; q-- > 0;)
+                glast_all[at + q] = (q + 1 < m && pc.cgid[q] == pc.cgid[q + 1]) ? glast_all[at + q + 1] : local + q;
+        }
+    });
+    pr.h_tasks = std::move(tasks);
+    mark("task_join");
     for (int cls = 0; cls < NUM_CLASSES; ++cls) pr.tile[cls] = ctx->tiles[precision][cls];
     STCHK(upload_blocks(ctx, pr));
-    mark("tasks_blocks");
+    mark("blocks");
     if (screen == FOCK_SCREEN_REF_BF_PAIR) {
         // basis-function pair products within 1e-12 of the reference's threshold (estimate)
         std::vector<double> dv;
@@ -971,16 +1107,19 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         HIPCHK(hipStreamSynchronize(S(ctx)));
     }
     if (precision == 1) {
-        const size_t nt = std::max<size_t>(tasks.size(), 1);
+        const size_t nt = std::max<size_t>(pr.h_tasks.size(), 1);
         HIPCHK(pr.d_gfirst.ensure(nt * sizeof(int)));
         HIPCHK(pr.d_glast.ensure(nt * sizeof(int)));
         HIPCHK(pr.d_Pm.ensure((size_t)ns * ns * sizeof(float)));
-        HIPCHK(hipMemcpyAsync(pr.d_gfirst.p, gfirst_all.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
-        HIPCHK(hipMemcpyAsync(pr.d_glast.p, glast_all.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_gfirst.p, gfirst_all.data(), pr.h_tasks.size() * sizeof(int), hipMemcpyHostToDevice,
+                              S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_glast.p, glast_all.data(), pr.h_tasks.size() * sizeof(int), hipMemcpyHostToDevice,
+                              S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
     }
-    HIPCHK(pr.d_tasks.ensure(std::max<size_t>(tasks.size(), 1) * sizeof(Task)));
-    HIPCHK(hipMemcpyAsync(pr.d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, S(ctx)));
+    HIPCHK(pr.d_tasks.ensure(std::max<size_t>(pr.h_tasks.size(), 1) * sizeof(Task)));
+    HIPCHK(hipMemcpyAsync(pr.d_tasks.p, pr.h_tasks.data(), pr.h_tasks.size() * sizeof(Task), hipMemcpyHostToDevice,
+                          S(ctx)));
     HIPCHK(pr.d_Jb.ensure((size_t)n * n * sizeof(double)));
     HIPCHK(pr.d_Kb.ensure((size_t)n * n * sizeof(double)));
     std::vector<int> f0(ns), nf(ns);
@@ -3144,62 +3283,113 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     }
     phase_mark("one_electron");
 
-    // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146)
-    HIPCHK(copy_f64(nn, S_, T1, S(ctx)));
-    STCHK(eig_sym(ctx, n, T1, W_, dWork.as<double>(), dInfo.as<int>()));  // T1 = R (rows = eigenvectors)
+    // X = U diag(1/sqrt((U^T S U)_jj)) U^T (scf/src/lib.rs:139-146), i.e. S^{-1/2}: by the coupled
+    // Newton-Schulz iteration on the matrix cores (three products per step, ~13 steps for the
+    // condition number ~2e3 of C20H42/6-31G), checked every 8 steps. When it has not converged
+    // after 64 steps (an ill-conditioned or singular overlap), through the eigenvectors as the
+    // reference builds it, with the singular-overlap check.
     out->n_electrons = n_electrons;
     out->n_basis = n;
-    STCHK(overlap_definite(ctx, n, W_));
-    HIPCHK(transpose_f64(n, T1, U_, S(ctx)));                               // U: columns = eigenvectors
-    HIPCHK(gemm_f64(false, false, n, n, n, 1.0, S_, n, U_, n, 0.0, T2, n, S(ctx)));   // S U
-    HIPCHK(gemm_f64(true, false, n, n, n, 1.0, U_, n, T2, n, 0.0, T1, n, S(ctx)));    // U^T S U
-    HIPCHK(scale_columns(n, U_, T1, n, T2, S(ctx)));                                   // U diag
-    HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U_, n, 0.0, X_, n, S(ctx)));    // (U diag) U^T
+    bool x_done = false;
+    {
+        constexpr int NS_CHUNK = 8, NS_MAX = 64;
+        DevBuf dRes, dBound;
+        HIPCHK(dRes.ensure(NS_MAX * sizeof(double)));
+        HIPCHK(dBound.ensure(2 * sizeof(double)));
+        double *res = dRes.as<double>(), *bound = dBound.as<double>();
+        HIPCHK(fill_f64(NS_MAX, 0.0, res, S(ctx)));
+        HIPCHK(gershgorin_bounds(n, S_, bound, S(ctx)));
+        double *Y = U_, *Y2 = C_, *Z = F_, *Z2 = Fx_;
+        HIPCHK(ns_init(n, S_, bound, Y, Z, S(ctx)));
+        for (int it = 0; it < NS_MAX && !x_done; it += NS_CHUNK) {
+            for (int k = 0; k < NS_CHUNK; ++k) {
+                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, Z, n, Y, n, 0.0, T1, n, S(ctx)));   // Z Y
+                HIPCHK(ns_update(n, T1, T2, res + it + k, S(ctx)));                             // T
+                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, Y, n, T2, n, 0.0, Y2, n, S(ctx)));  // Y T
+                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, T2, n, Z, n, 0.0, Z2, n, S(ctx)));  // T Z
+                std::swap(Y, Y2);
+                std::swap(Z, Z2);
+            }
+            double r = 0.0;
+            HIPCHK(hipMemcpyAsync(&r, res + it + NS_CHUNK - 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            HIPCHK(hipStreamSynchronize(S(ctx)));
+            if (!std::isfinite(r)) break;
+            // quadratic convergence: |I - ZY| below 1e-9 before the last step is ~1e-18 after it
+            x_done = r < 1e-9;
+        }
+        if (x_done) HIPCHK(ns_finish(n, Z, bound, X_, S(ctx)));
+    }
+    if (!x_done) {
+        HIPCHK(copy_f64(nn, S_, T1, S(ctx)));
+        STCHK(eig_sym(ctx, n, T1, W_, dWork.as<double>(), dInfo.as<int>()));  // T1 = R (rows = eigenvectors)
+        STCHK(overlap_definite(ctx, n, W_));
+        HIPCHK(transpose_f64(n, T1, U_, S(ctx)));                               // U: columns = eigenvectors
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, S_, n, U_, n, 0.0, T2, n, S(ctx)));   // S U
+        HIPCHK(gemm_f64(true, false, n, n, n, 1.0, U_, n, T2, n, 0.0, T1, n, S(ctx)));    // U^T S U
+        HIPCHK(scale_columns(n, U_, T1, n, T2, S(ctx)));                                   // U diag
+        HIPCHK(gemm_f64(false, true, n, n, n, 1.0, T2, n, U_, n, 0.0, X_, n, S(ctx)));    // (U diag) U^T
+    }
 
     // solve F' = X^T M X; C = X C'; P = 2 C_occ C_occ^T; eigenvalues in W_
     // Density of F' = X^T M X without orbitals (accurate policy, iterations that do not exit):
     // trace-correcting purification (Niklasson, Phys. Rev. B 66, 155115 (2002)). From
     // X0 = (hi - F')/(hi - lo) (Gershgorin bounds) each step takes X^2 or 2X - X^2, whichever
     // trace is closer to nocc, until tr X - tr X^2 = sum lambda (1 - lambda) vanishes; then
-    // P = 2 X Xp X^T. False (and no P) when it does not converge within MAX_STEPS (about twice
-    // log2(width / gap) steps are needed: a gap of 1e-4 of the width takes ~50, a gap at rounding
-    // level over 100), and when the whole
-    // spectrum is degenerate to working precision (width below 1e-8 of max|lambda|: the bounds
-    // would only measure rounding noise, e.g. the GWH guess of H2, a multiple of the identity).
-    // Those go to the eigensolver, whose choice within a degenerate space is the reference's.
-    DevBuf dXp, dX2, dPur;
+    // P = 2 X Xp X^T. False (and no P) when it does not converge within TC2_MAX_STEPS (about
+    // twice log2(width / gap) steps are needed: a gap of 1e-5 of the Gershgorin width takes ~66 --
+    // the GWH guess of C20H42/6-31G, 7.7e-4 Eh over 85 Eh --, a gap at rounding level over 100),
+    // and when the whole spectrum is degenerate to working precision (width below 1e-8 of
+    // max|lambda|: the bounds would only measure rounding noise, e.g. the GWH guess of H2, a
+    // multiple of the identity). Those go to the eigensolver, whose choice within a degenerate
+    // space is the reference's. Steps run in chunks without host round trips (one launch per
+    // step, the grow/shrink choice made on the device); the host looks at tr X - tr X^2 after each
+    // chunk.
+    constexpr int TC2_CHUNK = 8, TC2_MAX_STEPS = 96;
+    DevBuf dXp, dX2, dPur, dPart, dHist;
     const bool purify_ok = opts.integral_policy == FOCK_POLICY_ACCURATE;
+    const int npart_max = std::max(tc2_init_parts(n), tc2_step_parts(n));
     if (purify_ok) {
         HIPCHK(dXp.ensure(bytes));
         HIPCHK(dX2.ensure(bytes));
         HIPCHK(dPur.ensure(8 * sizeof(double)));
+        HIPCHK(dPart.ensure(4 * (size_t)npart_max * sizeof(double)));
+        HIPCHK(dHist.ensure(3 * TC2_MAX_STEPS * sizeof(double)));
     }
+    int tc2_last = 0;  // steps the previous purification took: the next one's first chunk
     auto purify = [&](bool &ok) -> fock_status {
         ok = false;
-        double *Xp = dXp.as<double>(), *X2 = dX2.as<double>(), *pur = dPur.as<double>();  // pur: trX, trX2, grow, -, lo, hi
+        double *pur = dPur.as<double>(), *hist = dHist.as<double>();  // pur[4..5]: lo, hi
+        double *xin = dXp.as<double>(), *xout = dX2.as<double>();
+        double *pin = dPart.as<double>(), *pout = pin + 2 * (size_t)npart_max;
         HIPCHK(gershgorin_bounds(n, T2, pur + 4, S(ctx)));
-        HIPCHK(tc2_init(n, T2, pur + 4, Xp, S(ctx)));
-        // steps run in chunks without host round trips (the grow/shrink choice is made on the
-        // device); the host looks at tr X - tr X^2 after each chunk
-        constexpr int CHUNK = 8, MAX_STEPS = 64;
-        for (int it = 0; it < MAX_STEPS && !ok; it += CHUNK) {
-            for (int k = 0; k < CHUNK; ++k) {
-                HIPCHK(gemm_f64(false, false, n, n, n, 1.0, Xp, n, Xp, n, 0.0, X2, n, S(ctx)));
-                HIPCHK(trace2(n, Xp, X2, (double)nocc, pur, S(ctx)));
-                HIPCHK(tc2_update((int64_t)n * n, pur + 2, Xp, X2, S(ctx)));
+        HIPCHK(tc2_init(n, T2, pur + 4, xin, pin, S(ctx)));
+        int npart = tc2_init_parts(n);
+        // consecutive SCF iterations need similar step counts: the first chunk runs one chunk
+        // short of the previous count (steps past convergence leave the projector as it is)
+        int chunk = std::max(TC2_CHUNK, tc2_last - TC2_CHUNK);
+        for (int it = 0; it < TC2_MAX_STEPS && !ok; it += chunk, chunk = TC2_CHUNK) {
+            chunk = std::min(chunk, TC2_MAX_STEPS - it);
+            for (int k = 0; k < chunk; ++k) {
+                HIPCHK(tc2_step(n, xin, xout, pin, npart, pout, (double)nocc, hist + 3 * (it + k), S(ctx)));
+                std::swap(xin, xout);
+                std::swap(pin, pout);
+                npart = tc2_step_parts(n);
             }
-            double h[6];
-            HIPCHK(hipMemcpyAsync(h, pur, 6 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            tc2_last = it + chunk;
+            double h[3], b[2];
+            HIPCHK(hipMemcpyAsync(h, hist + 3 * (it + chunk - 1), 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                  S(ctx)));
+            HIPCHK(hipMemcpyAsync(b, pur + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
-            if (!(h[5] - h[4] > 1e-8 * std::max(std::fabs(h[4]), std::fabs(h[5])))) return FOCK_OK;
-            const double idem = h[0] - h[1];  // of the step before the last update
+            if (!(b[1] - b[0] > 1e-8 * std::max(std::fabs(b[0]), std::fabs(b[1])))) return FOCK_OK;
+            const double idem = h[0] - h[1];  // of the input of the last step
             if (!std::isfinite(idem)) return FOCK_OK;
             // quadratic convergence: below 1e-9 before the last step means below 1e-18 after it
             if (idem < 1e-9) ok = std::fabs(h[0] - nocc) < 1e-8;
         }
         if (!ok) return FOCK_OK;
-        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, X_, n, Xp, n, 0.0, T1, n, S(ctx)));  // X Xp
-        HIPCHK(gemm_f64(false, true, n, n, n, 2.0, T1, n, X_, n, 0.0, P_, n, S(ctx)));  // 2 X Xp X^T
+        HIPCHK(gemm_f64(false, false, n, n, n, 1.0, X_, n, xin, n, 0.0, T1, n, S(ctx)));  // X Xp
+        HIPCHK(gemm_f64(false, true, n, n, n, 2.0, T1, n, X_, n, 0.0, P_, n, S(ctx)));   // 2 X Xp X^T
         return FOCK_OK;
     };
     // vectors: the orbitals and orbital energies are needed (the returned iteration); otherwise
@@ -3231,6 +3421,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         // GWH guess (scf/src/lib.rs:150-162)
         HIPCHK(gwh_guess(n, S_, H_, F_, S(ctx)));
         STCHK(diagonalize(F_, false));
+        tc2_last = 0;  // the guess's small gap is no measure for the iterations
     }
 
     phase_mark("guess");
@@ -3288,6 +3479,22 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             (void)hipEventDestroy(b);
         }
     } evg{ev0, ev1};
+    // Pinned staging: the DIIS / EDIIS coefficients and ring slots go up, and each iteration's
+    // energy comes down, without a synchronisation of their own. The copies of an iteration are
+    // ordered before the next iteration's synchronisation (after which the coefficient buffer is
+    // free again and the energy is read); the two energy slots alternate.
+    struct PinnedCoef {
+        void *h = nullptr;
+        ~PinnedCoef() {
+            if (h) (void)hipHostFree(h);
+        }
+    } pinc;
+    HIPCHK(hipHostMalloc(&pinc.h, 18 * sizeof(double) + 16 * sizeof(int), hipHostMallocDefault));
+    double *pin_e = static_cast<double *>(pinc.h);  // [2]
+    double *pin_coef = pin_e + 2;                    // [16]
+    int *pin_slots = reinterpret_cast<int *>(pin_coef + 16);
+    std::vector<double> e_host;  // electronic energy of every iteration, read one iteration late
+    int last_e = -1;             // last iteration whose energy was computed
     out->history_len = 0;
     fock_status result = FOCK_NOT_CONVERGED;
     for (int iter = 0; iter <= opts.max_iters; ++iter) {
@@ -3369,6 +3576,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             HIPCHK(hipMemcpyAsync(hd + 16, scal, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             if (ediis_on) HIPCHK(hipMemcpyAsync(he, dEd.p, 33 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
+            if (last_e == iter - 1 && (int)e_host.size() == last_e) e_host.push_back(pin_e[last_e & 1]);
             for (int i = 0; i < m; ++i) newdots[i] = hd[slots[i]];
             out->last_error = hd[16];
             if (ediis_on) {  // the new row (P_new F_j) and column (P_j F_new) of tr(P_i F_j)
@@ -3393,6 +3601,17 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         // element) the coefficients, and so the returned orbitals and energy (measured: up to
         // 7e-6 Eh), follow the rounding of the last Fock build.
         const double *Fuse = f_slot;
+        // Fx = sum_i c_i F_(slot i) over the stored matrices
+        auto upload_combination = [&](const std::vector<double> &c) -> fock_status {
+            for (int i = 0; i < m; ++i) {
+                pin_coef[i] = c[i];
+                pin_slots[i] = slots[i];
+            }
+            HIPCHK(hipMemcpyAsync(dCoef.p, pin_coef, m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
+            HIPCHK(hipMemcpyAsync(dSlots.p, pin_slots, m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+            HIPCHK(lincomb(nn, dFocks.as<double>(), nn, dSlots.as<int>(), dCoef.as<double>(), m, Fx_, S(ctx)));
+            return FOCK_OK;
+        };
         const bool exit_plain = opts.integral_policy == FOCK_POLICY_ACCURATE && phase == 1 && !incr_build &&
                                 err < opts.epsilon;
         bool extrapolated = false;
@@ -3405,11 +3624,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 for (int j = 0; j < m; ++j)
                     Q[(size_t)i * m + j] = -0.5 * (pf[i][i] + pf[j][j] - pf[i][j] - pf[j][i]);
             if (simplex_qp(m, ev, Q, c)) {
-                std::vector<int> sl(slots.begin(), slots.end());
-                HIPCHK(hipMemcpyAsync(dCoef.p, c.data(), m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
-                HIPCHK(hipMemcpyAsync(dSlots.p, sl.data(), m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
-                HIPCHK(lincomb(nn, dFocks.as<double>(), nn, dSlots.as<int>(), dCoef.as<double>(), m, Fx_, S(ctx)));
-                HIPCHK(hipStreamSynchronize(S(ctx)));  // c, sl are host temporaries
+                STCHK(upload_combination(c));
                 Fuse = Fx_;
                 extrapolated = true;
             }
@@ -3419,11 +3634,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             for (int i = 0; i < m; ++i)
                 for (int j = 0; j < m; ++j) Bm[(size_t)i * m + j] = dots[i][j];
             if (diis_solve(m, Bm, c)) {
-                std::vector<int> sl(slots.begin(), slots.end());
-                HIPCHK(hipMemcpyAsync(dCoef.p, c.data(), m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
-                HIPCHK(hipMemcpyAsync(dSlots.p, sl.data(), m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
-                HIPCHK(lincomb(nn, dFocks.as<double>(), nn, dSlots.as<int>(), dCoef.as<double>(), m, Fx_, S(ctx)));
-                HIPCHK(hipStreamSynchronize(S(ctx)));  // c, sl are host temporaries
+                STCHK(upload_combination(c));
                 Fuse = Fx_;
             }
         }
@@ -3447,24 +3658,22 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         if (damp) HIPCHK(mix_f64(nn, opts.damping, P_, dPold.as<double>(), P_, S(ctx)));
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
         HIPCHK(energy_trace(n, P_, H_, G_, scal + 1, S(ctx)));
-        double e_elec = 0.0;
-        HIPCHK(hipMemcpyAsync(&e_elec, scal + 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
-        HIPCHK(hipStreamSynchronize(S(ctx)));
+        HIPCHK(hipMemcpyAsync(pin_e + (iter & 1), scal + 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+        last_e = iter;
         phase_mark("post_eig");
         float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));  // ev1 precedes this iteration's synchronisation
         if (out->history && out->history_len < out->history_capacity) {
-            fock_scf_iter &h = out->history[out->history_len++];
-            h.total_energy = e_elec + e_nuc;
+            fock_scf_iter &h = out->history[out->history_len++];  // the energy is filled in at the end
             h.error = err;
             h.fock_build_ms = ms;
             h.quartets = evaluated;
         }
-        if (debug)
+        if (debug) {
+            HIPCHK(hipStreamSynchronize(S(ctx)));
             std::fprintf(stderr, "[fock] iter %d phase %d err %.6e E %.12f build %.3f ms quartets %lld\n", iter, phase,
-                         err, e_elec + e_nuc, ms, (long long)evaluated);
-        out->electronic_energy = e_elec;
-        out->total_energy = e_elec + e_nuc;
+                         err, pin_e[iter & 1] + e_nuc, ms, (long long)evaluated);
+        }
         out->iterations = iter;
         if (phase == 0) {
             // mixed phase: hand over to FP64 full builds once near convergence or stalled; the
@@ -3516,6 +3725,13 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     }
     out->n_electrons = n_electrons;
     out->n_basis = n;
+    HIPCHK(hipStreamSynchronize(S(ctx)));
+    if (last_e >= 0) {  // the energies of the iterations (history) and of the last one
+        if ((int)e_host.size() == last_e) e_host.push_back(pin_e[last_e & 1]);
+        for (int k = 0; k < out->history_len && k <= last_e; ++k) out->history[k].total_energy = e_host[k] + e_nuc;
+        out->electronic_energy = e_host[last_e];
+        out->total_energy = e_host[last_e] + e_nuc;
+    }
     if (result == FOCK_DIVERGED) {
         ctx->err = "SCF diverged: max|FDS - SDF| is not a normal number";
         return result;

@@ -417,9 +417,19 @@ hipError_t copy_f64(int64_t len, const double *in, double *out, hipStream_t s);
 // X = (hi I - A)/(hi - lo), out = {tr X, tr X2, grow} (grow: tr(2X - X2) closer to target than
 // tr X2), X = 2X - X2 if *grow else X2 (the choice stays on the device)
 hipError_t gershgorin_bounds(int n, const double *A, double *out, hipStream_t s);
-hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, hipStream_t s);
-hipError_t trace2(int n, const double *X, const double *X2, double target, double *out, hipStream_t s);
-hipError_t tc2_update(int64_t len, const double *grow, double *X, const double *X2, hipStream_t s);
+// Newton-Schulz S^{-1/2}: Y = S / bound[1], Z = I; T = (3 I - ZY) / 2 with *res = max(*res, max|I - ZY|);
+// X = (Z + Z^T) / (2 sqrt(bound[1]))
+hipError_t ns_init(int n, const double *S, const double *bound, double *Y, double *Z, hipStream_t s);
+hipError_t ns_update(int n, const double *ZY, double *T, double *res, hipStream_t s);
+hipError_t ns_finish(int n, const double *Z, const double *bound, double *X, hipStream_t s);
+// X = (hi I - A) / (hi - lo), per-workgroup (tr, sum of squares) partials in part[tc2_init_parts(n)][2]
+int tc2_init_parts(int n);
+hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, double *part, hipStream_t s);
+// one purification step X -> Xout (X^2 or 2X - X^2, chosen on the device from part_in), the new
+// partials in part_out[tc2_step_parts(n)][2], hist = {tr X, tr X^2, grow} of the input
+int tc2_step_parts(int n);
+hipError_t tc2_step(int n, const double *X, double *Xout, const double *part_in, int npart_in, double *part_out,
+                    double nocc, double *hist, hipStream_t s);
 // GWH: out_ij = 0.875 S_ij (H_ii + H_jj)
 hipError_t gwh_guess(int n, const double *S, const double *H, double *out, hipStream_t s);
 // X = U diag(1/sqrt(D_ii)) U^T with D = U^T S U: scale columns of U by dinv (device vector)
