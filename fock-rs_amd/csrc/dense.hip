@@ -363,6 +363,43 @@ hipError_t shell_block_maxabs(int nshell, const int *shell_f0, const int *shell_
     return hipGetLastError();
 }
 
+__global__ void pair_block_max_kernel(int npair, const int2 *__restrict__ pair_shell, const float *__restrict__ Pm,
+                                      int nshell, float *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npair) return;
+    const int2 ab = pair_shell[j];
+    out[j] = Pm[(int64_t)ab.x * nshell + ab.y];
+}
+hipError_t pair_block_max(int npair, const int2 *pair_shell, const float *Pm, int nshell, float *out, hipStream_t s) {
+    if (npair <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_block_max_kernel, dim3((npair + 255) / 256), dim3(256), 0, s, npair, pair_shell, Pm, nshell,
+                       out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(1024) void max_f32_kernel(int64_t len, const float *__restrict__ x, float *__restrict__ out) {
+    __shared__ float red[1024];
+    float m = 0.0f;
+    for (int64_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const float v = x[i];
+        m = (v != v || v > m) ? v : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const float o = red[threadIdx.x + off], c = red[threadIdx.x];
+            red[threadIdx.x] = (o != o || o > c) ? o : c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+hipError_t max_f32(int64_t len, const float *x, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(max_f32_kernel, dim3(1), dim3(1024), 0, s, len, x, out);
+    return hipGetLastError();
+}
+
 __global__ void transpose_kernel(int n, const double *__restrict__ in, double *__restrict__ out) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * n) return;

@@ -511,6 +511,10 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         blk_ = r.blocks[bb.y];
     } else {
         blk_ = args.blocks[blockIdx.x];
+        if (args.block_qmax) {  // density screening: the whole block below the bound returns
+            const float ub = args.pairQ[blk_.bra] * args.block_qmax[blockIdx.x] * (2.0f * *args.pm_max);
+            if (!(ub >= args.bound_lo)) return;  // the same for every thread of the block
+        }
     }
     const Block blk = blk_;
     const int n = args.nbf;  // LDS is sized for the largest basis of the launch
@@ -524,6 +528,9 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     Task *s_task = reinterpret_cast<Task *>(red + 4 * NAB);         // [tile]
     int *s_kidx = reinterpret_cast<int *>(s_task + tile);  // [tile * 64] kept ket pair indices
     unsigned long long *s_mask = reinterpret_cast<unsigned long long *>(s_kidx + tile * 64);  // [tile]
+    // density screening: the shell-block maxima of the bra shells' rows (the lane bound reads them)
+    float *s_pma = reinterpret_cast<float *>(s_mask + tile);  // [nshell]
+    float *s_pmb = s_pma + args.nshell;                        // [nshell]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
@@ -531,6 +538,16 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     if (threadIdx.x < blk.ntasks) {
         const Task tk = args.tasks[blk.first_task + threadIdx.x];
         s_task[threadIdx.x] = tk;
+    }
+    float pm_ab = 0.0f, q_ab = 0.0f;
+    if (args.pair_shell) {
+        const int2 ab = args.pair_shell[blk.bra];
+        for (int e = threadIdx.x; e < args.nshell; e += blockDim.x) {
+            s_pma[e] = args.Pm[(size_t)ab.x * args.nshell + e];
+            s_pmb[e] = args.Pm[(size_t)ab.y * args.nshell + e];
+        }
+        pm_ab = args.Pm[(size_t)ab.x * args.nshell + ab.y];
+        q_ab = args.pairQ[blk.bra];
     }
     __syncthreads();
     // the block's kept kets, packed densely in task order: wave w owns task chunks w, w + 4 (a
@@ -540,15 +557,14 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         const Task tk = s_task[t];
         bool keep = lane < tk.ket_count;
         if (args.pair_shell && keep) {
-            // Q_ab Q_cd max(2|P_ab|, 2|P_cd|, |P_ac|, |P_ad|, |P_bc|, |P_bd|) over shell blocks
+            // Q_ab Q_cd max(2|P_ab|, 2|P_cd|, |P_ac|, |P_ad|, |P_bc|, |P_bd|) over shell blocks: the
+            // ket's own block by pair (coalesced), the bra rows from LDS
             const int ket_idx = tk.ket_begin + lane;
-            const int2 ab = args.pair_shell[blk.bra], cd = args.pair_shell[ket_idx];
-            const float *Pm = args.Pm;
-            const int ns = args.nshell;
-            float pm = 2.0f * fmaxf(Pm[ab.x * ns + ab.y], Pm[cd.x * ns + cd.y]);
-            pm = fmaxf(pm, fmaxf(Pm[ab.x * ns + cd.x], Pm[ab.x * ns + cd.y]));
-            pm = fmaxf(pm, fmaxf(Pm[ab.y * ns + cd.x], Pm[ab.y * ns + cd.y]));
-            const float bound = args.pairQ[blk.bra] * args.pairQ[ket_idx] * pm;
+            const int2 cd = args.pair_shell[ket_idx];
+            float pm = 2.0f * fmaxf(pm_ab, args.Pm_pair[ket_idx]);
+            pm = fmaxf(pm, fmaxf(s_pma[cd.x], s_pma[cd.y]));
+            pm = fmaxf(pm, fmaxf(s_pmb[cd.x], s_pmb[cd.y]));
+            const float bound = q_ab * args.pairQ[ket_idx] * pm;
             keep = bound >= args.bound_lo && bound < args.bound_hi;
         } else if (args.ket_tau > 0.0f && keep) {
             keep = args.pairQ[blk.bra] * args.pairQ[tk.ket_begin + lane] >= args.ket_tau;
@@ -559,6 +575,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     __syncthreads();
     int kept = 0;
     for (int t = 0; t < blk.ntasks; ++t) kept += __popcll(s_mask[t]);
+    if (args.kept_count && threadIdx.x == 0 && kept > 0) atomicAdd(args.kept_count, (unsigned long long)kept);
     if (kept == 0) return;  // the same for every thread of the block
     // every kept ket is written to its rank among the block's kept kets; the K rows, the bra
     // primitive pairs with their q and this class's Boys slab are staged
@@ -748,7 +765,7 @@ static hipError_t launch_block(const JKArgs &a, hipStream_t s) {
     if (a.nblocks <= 0) return hipSuccess;
     if (a.tile < 1 || a.tile > MAX_TILE) return hipErrorInvalidValue;
     constexpr int cls = (LA + LB == 0) ? 0 : (LA + LB == 1) ? (LC + LD == 0 ? 1 : 2) : (LC + LD == 0 ? 3 : LC + LD == 1 ? 4 : 5);
-    const size_t lds = block_lds_dynamic(cls, a.nbf, a.tile);
+    const size_t lds = block_lds_dynamic(cls, a.nbf, a.tile) + (a.pair_shell ? 2 * (size_t)a.nshell * sizeof(float) : 0);
     hipLaunchKernelGGL((eri_block_kernel<LA, LB, LC, LD, POLICY, REFSCREEN, Real, BATCH>), dim3(a.nblocks), dim3(256),
                        lds, s, a);
     return hipGetLastError();

@@ -121,6 +121,9 @@ struct Prepared {
     DevBuf d_shell_f0, d_shell_nf;
     // incremental builds: per-task group bounds (class-local), pair shells and bounds, buffers
     DevBuf d_gfirst, d_glast, d_pair_shell, d_pairQ, d_Pm, d_flag32, d_flag64;
+    DevBuf d_block_qmax, d_pm_max;  // per block: largest ket Q (blocked density screening); max Pm
+    DevBuf d_kept;                  // kets evaluated by a blocked density-screened build
+    DevBuf d_Pm_pair;               // Pm of every shell pair's own block, by pair
     struct Compact {
         DevBuf pos, kets, head, tid, kbra, kend, kcost, tasks, count, tmp;
         size_t tmp_bytes = 0;
@@ -658,6 +661,22 @@ fock_status upload_blocks(fock_ctx *ctx, Prepared &pr) {
     HIPCHK(pr.d_blocks.ensure(std::max<size_t>(pr.h_tasks.size(), 1) * sizeof(Block)));
     HIPCHK(hipMemcpyAsync(pr.d_blocks.p, pr.h_blocks.data(), pr.h_blocks.size() * sizeof(Block),
                           hipMemcpyHostToDevice, S(ctx)));
+    // largest Q = sqrt(dmax) of every block's kets: the first ket of each task (kets of a task are
+    // consecutive in descending dmax)
+    std::vector<float> qmax(pr.h_blocks.size());
+    for (int cls = 0; cls < NUM_CLASSES; ++cls)
+        for (int b = pr.block_off[cls]; b < pr.block_off[cls + 1]; ++b) {
+            const Block &bk = pr.h_blocks[b];
+            float m = 0.0f;
+            for (int t = 0; t < bk.ntasks; ++t) {
+                const Task &tk = pr.h_tasks[pr.task_off[cls] + bk.first_task + t];
+                m = std::max(m, (float)std::sqrt(std::max(pr.pairs[tk.ket_begin].dmax, 0.0)));
+            }
+            qmax[b] = m;
+        }
+    HIPCHK(pr.d_block_qmax.ensure(std::max<size_t>(pr.h_tasks.size(), 1) * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(pr.d_block_qmax.p, qmax.data(), qmax.size() * sizeof(float), hipMemcpyHostToDevice,
+                          S(ctx)));
     HIPCHK(hipStreamSynchronize(S(ctx)));  // h_blocks may be rebuilt right after
     return FOCK_OK;
 }
@@ -1227,13 +1246,19 @@ hipError_t scatter_pair_j(fock_ctx *ctx, double *Jb) {
 }
 
 // The blocked digestion applies when every class's workgroup, at its tile, fits the device's LDS.
-bool blocked_fits(fock_ctx *ctx, int n, bool fp32, int policy, const int *tile) {
+bool blocked_fits(fock_ctx *ctx, int n, bool fp32, int policy, const int *tile, size_t extra = 0) {
     if (n > BLOCKED_MAX_NBF) return false;
     for (int c = 0; c < NUM_CLASSES; ++c)
-        if (block_lds_bytes(c, n, policy, fp32, tile[c]) > ctx->max_lds) return false;
+        if (block_lds_bytes(c, n, policy, fp32, tile[c]) + extra > ctx->max_lds) return false;
     return true;
 }
 bool blocked_fits(fock_ctx *ctx, int n, bool fp32) { return blocked_fits(ctx, n, fp32, ctx->jk.policy, ctx->jk.tile); }
+// the density-screened blocked build also holds two rows of shell-block maxima in LDS
+bool blocked_fits_screened(fock_ctx *ctx, int n) {
+    const size_t rows = 2 * ctx->jk.shells.size() * sizeof(float);
+    return blocked_fits(ctx, n, true, ctx->jk.policy, ctx->jk.tile, rows) &&
+           blocked_fits(ctx, n, false, ctx->jk.policy, ctx->jk.tile, rows);
+}
 
 // Kernel arguments shared by every build of class cls over the prepared task lists.
 JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *Jb, double *Kb) {
@@ -1322,12 +1347,23 @@ CompactBuffers view(Prepared::Compact &c) {
 // Density-weighted incremental build on the static blocks: J/K of dP over the quartets whose
 // bound Q_ab Q_cd max|dP| reaches tau; bounds >= tau64 run in FP64, the rest in FP32.
 fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, float tau,
-                                           float tau64) {
+                                           float tau64, bool count_survivors) {
     Prepared &pr = ctx->jk;
     const int n = pr.n, ns = (int)pr.shells.size();
+    unsigned long long *kept = nullptr;
+    if (count_survivors) {
+        HIPCHK(pr.d_kept.ensure(sizeof(unsigned long long)));
+        kept = pr.d_kept.as<unsigned long long>();
+        HIPCHK(fill_f64(1, 0.0, reinterpret_cast<double *>(kept), S(ctx)));  // all-zero bits
+    }
     HIPCHK(fill_f64((int64_t)n * n, 0.0, Jb, S(ctx)));
     HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
     HIPCHK(shell_block_maxabs(ns, pr.d_shell_f0.as<int>(), pr.d_shell_nf.as<int>(), n, dP, pr.d_Pm.as<float>(), S(ctx)));
+    HIPCHK(pr.d_pm_max.ensure(sizeof(float)));
+    HIPCHK(max_f32((int64_t)ns * ns, pr.d_Pm.as<float>(), pr.d_pm_max.as<float>(), S(ctx)));
+    const int npair = (int)pr.pairs.size();
+    HIPCHK(pr.d_Pm_pair.ensure(std::max(npair, 1) * sizeof(float)));
+    HIPCHK(pair_block_max(npair, pr.d_pair_shell.as<int2>(), pr.d_Pm.as<float>(), ns, pr.d_Pm_pair.as<float>(), S(ctx)));
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         for (int prec = 0; prec < 2; ++prec) {
             if (prec == 1 && !(tau64 < HUGE_VALF)) continue;  // FP32 only
@@ -1336,6 +1372,10 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
             a.pairQ = pr.d_pairQ.as<float>();
             a.Pm = pr.d_Pm.as<float>();
             a.nshell = ns;
+            a.Pm_pair = pr.d_Pm_pair.as<float>();
+            a.block_qmax = pr.d_block_qmax.as<float>() + pr.block_off[cls];
+            a.pm_max = pr.d_pm_max.as<float>();
+            a.kept_count = kept;
             a.bound_lo = prec == 0 ? tau : tau64;
             a.bound_hi = prec == 0 ? tau64 : HUGE_VALF;
             HIPCHK(launch_eri_blocks(cls, a, pr.policy, false, prec == 0, S(ctx)));
@@ -3509,9 +3549,15 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             }
             HIPCHK(sub_f64(nn, P_, Pprev, dPinc, S(ctx)));
             const float tau = phase == 0 ? INC_TAU_BULK : INC_TAU, tau64 = phase == 0 ? HUGE_VALF : INC_TAU64;
-            if (blocked_fits(ctx, n, true) && blocked_fits(ctx, n, false)) {
+            if (blocked_fits_screened(ctx, n)) {
                 STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau,
-                                                     tau64));
+                                                     tau64, want_counts));
+                if (want_counts) {
+                    unsigned long long kc = 0;
+                    HIPCHK(hipMemcpyAsync(&kc, pr.d_kept.p, sizeof(kc), hipMemcpyDeviceToHost, S(ctx)));
+                    HIPCHK(hipStreamSynchronize(S(ctx)));
+                    evaluated = (int64_t)kc;
+                }
             } else {
                 STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau, tau64,
                                              want_counts));

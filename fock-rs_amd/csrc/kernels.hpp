@@ -39,8 +39,14 @@ struct JKArgs {
     const int2 *pair_shell;  // null: no density screening
     const float *pairQ;
     const float *Pm;         // [nshell][nshell] max |P| per shell block
+    const float *Pm_pair;    // blocked kernel: Pm of every shell pair's own block, by pair
     int nshell;
     float bound_lo, bound_hi;
+    // blocked kernel with density screening: a block whose bound pairQ[bra] block_qmax 2 pm_max
+    // (an upper bound of every lane's) lies below bound_lo returns at once (null: no block test)
+    const float *block_qmax;  // per block: the largest Q of its kets
+    const float *pm_max;      // max over all shell blocks of Pm
+    unsigned long long *kept_count;  // blocked kernel: += the kets each block evaluates (null: none)
     // without density screening: a lane is evaluated only when Q_ab Q_cd >= ket_tau (0: all;
     // the FP32 full builds cut at their primitive threshold)
     float ket_tau;
@@ -436,6 +442,10 @@ hipError_t gwh_guess(int n, const double *S, const double *H, double *out, hipSt
 hipError_t scale_columns(int n, const double *U, const double *dvec_diag_source, int ld_diag, double *out,
                          hipStream_t s);
 // per shell-pair max |P| blocks for density-weighted screening
+// out[j] = Pm[pair_shell[j].x][pair_shell[j].y] for every shell pair j
+hipError_t pair_block_max(int npair, const int2 *pair_shell, const float *Pm, int nshell, float *out, hipStream_t s);
+// *out = max of x[0..len) (non-negative values; NaN propagates), one workgroup
+hipError_t max_f32(int64_t len, const float *x, float *out, hipStream_t s);
 hipError_t shell_block_maxabs(int nshell, const int *shell_f0, const int *shell_nf, int n, const double *P,
                               float *out, hipStream_t s);
 
