@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true", help="skip both SCF runs (builds use a synthetic density)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batching, gradient and orbital-grid forms (builds and the C20H42 SCFs only)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile (8 tasks per workgroup)")
     ap.add_argument("--tiles", default="", metavar="JSON",
                     help='set the tiles instead of tuning, e.g. the "tiles" of an earlier run\'s config: '
@@ -321,6 +323,10 @@ def main():
             # per iteration: build ms and the shell quartets it evaluated
             "builds": [[round(h[2], 3), int(h[3])] for h in r["history"]],
         }
+        # evaluated quartets over the full Schwarz list of the FP64 build, per iteration
+        full = max(int(h[3]) for h in r["history"]) if r["history"] else 0
+        if full:
+            scf[name]["kept_fraction"] = [round(int(h[3]) / full, 3) for h in r["history"]]
         if e_oracle is not None:
             scf[name]["abs_dE_vs_fp64_oracle"] = abs(r["total_energy"] - e_oracle)
         if name == "fp64":
@@ -410,7 +416,7 @@ def main():
         # build of the mixed SCF (FP32 bulk + incremental builds + FP64 cleanup builds)
         result["scf_build_ratio_fp64_over_mixed"] = scf["fp64"]["fock_build_ms_mean"] / scf["mixed"]["fock_build_ms_mean"]
         result["scf_mixed_mean_build_over_full_fp32"] = scf["mixed"]["fock_build_ms_mean"] / mixed["ms"]
-    if not args.no_scf:
+    if not args.no_scf and not args.no_forms:
         result["batch_form"] = batch_form(ctx, abi, m, np)
         log("batch form: %s" % result["batch_form"])
         result["gradient_form"] = gradient_form(ctx, abi, m, np)
