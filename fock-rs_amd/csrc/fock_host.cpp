@@ -62,6 +62,13 @@ double prim_threshold_grad(const fock_basis_desc *b) {
 // ket of the task) is below this run in FP32, the others in FP64
 constexpr float GRAD_TAU64 = 1e-4f;
 constexpr int CLASS_NINT[NUM_CLASSES] = {1, 3, 9, 9, 27, 81};
+// The class kernels of a blocked build are independent (their J/K contributions meet only in
+// atomics), so they run on CLASS_STREAMS streams at once: one kernel's tail and its idle issue
+// slots are filled by the others' workgroups. Stream of each class: the two largest classes alone,
+// the four smaller ones in pairs; the launch order puts the most unbalanced kernel, (pp|pp), first.
+constexpr int CLASS_STREAMS = 4;
+constexpr int CLASS_LANE[NUM_CLASSES] = {0, 1, 2, 3, 2, 3};
+constexpr int CLASS_LAUNCH_ORDER[NUM_CLASSES] = {5, 0, 1, 2, 3, 4};
 
 struct DevBuf {
     void *p = nullptr;
@@ -198,6 +205,9 @@ struct fock_ctx {
     int batch_schedule = FOCK_SCHEDULE_COST;  // block order of batched builds
     bool timing = false;
     hipEvent_t tev[NUM_CLASSES + 1] = {};
+    // side streams of the concurrent class launches and their fork/join events (made on first use)
+    hipStream_t side[CLASS_STREAMS - 1] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[CLASS_STREAMS - 1] = {};
     std::mutex err_mu;  // preparation threads of a batch may fail at once
     std::atomic<int64_t> preparations{0};  // task-list preparations made by this context
     fock_status fail(fock_status st, const std::string &msg) {
@@ -453,6 +463,13 @@ extern "C" void fock_ctx_destroy(fock_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (hipEvent_t &e : ctx->tev)
         if (e) (void)hipEventDestroy(e);
+    for (hipStream_t &s : ctx->side)
+        if (s) (void)hipStreamSynchronize(s);
+    for (hipStream_t &s : ctx->side)
+        if (s) (void)hipStreamDestroy(s);
+    for (hipEvent_t &e : ctx->join_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -1294,6 +1311,43 @@ JKArgs class_args(fock_ctx *ctx, int cls, bool fp32, const double *dP, double *J
     return a;
 }
 
+// The streams of the class launches of one blocked build: the context's stream and its side
+// streams, forked from and joined back into the context's stream. Serial (every class on the
+// context's stream, in class order) while per-class timing is on, so that each class's events
+// bracket its own kernel.
+struct ClassStreams {
+    fock_ctx *ctx = nullptr;
+    bool concurrent = false;
+    hipStream_t of(int cls) const {
+        const int lane = CLASS_LANE[cls];
+        return concurrent && lane > 0 ? ctx->side[lane - 1] : S(ctx);
+    }
+    int order(int k) const { return concurrent ? CLASS_LAUNCH_ORDER[k] : k; }
+};
+
+fock_status fork_classes(fock_ctx *ctx, ClassStreams &cs) {
+    cs.ctx = ctx;
+    cs.concurrent = !ctx->timing;
+    if (!cs.concurrent) return FOCK_OK;
+    for (int i = 0; i < CLASS_STREAMS - 1; ++i) {
+        if (!ctx->side[i]) HIPCHK(hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
+        if (!ctx->join_ev[i]) HIPCHK(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
+    }
+    if (!ctx->fork_ev) HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ctx->fork_ev, S(ctx)));
+    for (int i = 0; i < CLASS_STREAMS - 1; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0));
+    return FOCK_OK;
+}
+
+fock_status join_classes(fock_ctx *ctx, const ClassStreams &cs) {
+    if (!cs.concurrent) return FOCK_OK;
+    for (int i = 0; i < CLASS_STREAMS - 1; ++i) {
+        HIPCHK(hipEventRecord(ctx->join_ev[i], ctx->side[i]));
+        HIPCHK(hipStreamWaitEvent(S(ctx), ctx->join_ev[i], 0));
+    }
+    return FOCK_OK;
+}
+
 // Enqueue the six class kernels accumulating into Jb/Kb (zeroed here). fp32 selects the FP32
 // quartet kernels (mixed form); default: the prepared precision.
 fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, int mode, double *dense,
@@ -1306,15 +1360,19 @@ fock_status enqueue_jk(fock_ctx *ctx, const double *dP, double *Jb, double *Kb, 
     }
     const bool fp32 = (pr.precision == 1) && mode == MODE_JK && !force_fp64;
     const bool blocked = mode == MODE_JK && blocked_fits(ctx, n, fp32);
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+    ClassStreams cs;
+    if (blocked) STCHK(fork_classes(ctx, cs));
+    for (int k = 0; k < NUM_CLASSES; ++k) {
+        const int cls = cs.order(k);
         JKArgs a = class_args(ctx, cls, fp32, dP, Jb, Kb);
         a.dense = dense;
         if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[cls], S(ctx)));
         if (blocked)
-            HIPCHK(launch_eri_blocks(cls, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
+            HIPCHK(launch_eri_blocks(cls, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, cs.of(cls)));
         else
             HIPCHK(launch_eri_tasks(cls, mode, a, pr.policy, pr.screen == FOCK_SCREEN_REF_BF_PAIR, fp32, S(ctx)));
     }
+    if (blocked) STCHK(join_classes(ctx, cs));
     if (blocked) HIPCHK(scatter_pair_j(ctx, Jb));
     if (ctx->timing) HIPCHK(hipEventRecord(ctx->tev[NUM_CLASSES], S(ctx)));
     return FOCK_OK;
@@ -1364,7 +1422,10 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
     const int npair = (int)pr.pairs.size();
     HIPCHK(pr.d_Pm_pair.ensure(std::max(npair, 1) * sizeof(float)));
     HIPCHK(pair_block_max(npair, pr.d_pair_shell.as<int2>(), pr.d_Pm.as<float>(), ns, pr.d_Pm_pair.as<float>(), S(ctx)));
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+    ClassStreams cs;
+    STCHK(fork_classes(ctx, cs));
+    for (int k = 0; k < NUM_CLASSES; ++k) {
+        const int cls = cs.order(k);
         for (int prec = 0; prec < 2; ++prec) {
             if (prec == 1 && !(tau64 < HUGE_VALF)) continue;  // FP32 only
             JKArgs a = class_args(ctx, cls, prec == 0, dP, Jb, Kb);
@@ -1378,9 +1439,10 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
             a.kept_count = kept;
             a.bound_lo = prec == 0 ? tau : tau64;
             a.bound_hi = prec == 0 ? tau64 : HUGE_VALF;
-            HIPCHK(launch_eri_blocks(cls, a, pr.policy, false, prec == 0, S(ctx)));
+            HIPCHK(launch_eri_blocks(cls, a, pr.policy, false, prec == 0, cs.of(cls)));
         }
     }
+    STCHK(join_classes(ctx, cs));
     HIPCHK(scatter_pair_j(ctx, Jb));
     return FOCK_OK;
 }
