@@ -129,7 +129,7 @@ struct Prepared {
     // incremental builds: per-task group bounds (class-local), pair shells and bounds, buffers
     DevBuf d_gfirst, d_glast, d_pair_shell, d_pairQ, d_Pm, d_flag32, d_flag64;
     DevBuf d_block_qmax, d_pm_max;  // per block: largest ket Q (blocked density screening); max Pm
-    DevBuf d_kept;                  // kets evaluated by a blocked density-screened build
+    DevBuf d_kept;                  // kets evaluated by a blocked density-screened build, per class
     DevBuf d_Pm_pair;               // Pm of every shell pair's own block, by pair
     struct Compact {
         DevBuf pos, kets, head, tid, kbra, kend, kcost, tasks, count, tmp;
@@ -1410,9 +1410,9 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
     const int n = pr.n, ns = (int)pr.shells.size();
     unsigned long long *kept = nullptr;
     if (count_survivors) {
-        HIPCHK(pr.d_kept.ensure(sizeof(unsigned long long)));
+        HIPCHK(pr.d_kept.ensure(NUM_CLASSES * sizeof(unsigned long long)));
         kept = pr.d_kept.as<unsigned long long>();
-        HIPCHK(fill_f64(1, 0.0, reinterpret_cast<double *>(kept), S(ctx)));  // all-zero bits
+        HIPCHK(fill_f64(NUM_CLASSES, 0.0, reinterpret_cast<double *>(kept), S(ctx)));  // all-zero bits
     }
     HIPCHK(fill_f64((int64_t)n * n, 0.0, Jb, S(ctx)));
     HIPCHK(fill_f64((int64_t)n * n, 0.0, Kb, S(ctx)));
@@ -1436,7 +1436,7 @@ fock_status enqueue_jk_incremental_blocked(fock_ctx *ctx, const double *dP, doub
             a.Pm_pair = pr.d_Pm_pair.as<float>();
             a.block_qmax = pr.d_block_qmax.as<float>() + pr.block_off[cls];
             a.pm_max = pr.d_pm_max.as<float>();
-            a.kept_count = kept;
+            a.kept_count = kept ? kept + cls : nullptr;
             a.bound_lo = prec == 0 ? tau : tau64;
             a.bound_hi = prec == 0 ? tau64 : HUGE_VALF;
             HIPCHK(launch_eri_blocks(cls, a, pr.policy, false, prec == 0, cs.of(cls)));
@@ -3615,10 +3615,14 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau,
                                                      tau64, want_counts));
                 if (want_counts) {
-                    unsigned long long kc = 0;
-                    HIPCHK(hipMemcpyAsync(&kc, pr.d_kept.p, sizeof(kc), hipMemcpyDeviceToHost, S(ctx)));
+                    unsigned long long kc[NUM_CLASSES] = {};
+                    HIPCHK(hipMemcpyAsync(kc, pr.d_kept.p, sizeof(kc), hipMemcpyDeviceToHost, S(ctx)));
                     HIPCHK(hipStreamSynchronize(S(ctx)));
-                    evaluated = (int64_t)kc;
+                    evaluated = 0;
+                    for (int c = 0; c < NUM_CLASSES; ++c) evaluated += (int64_t)kc[c];
+                    if (debug)
+                        std::fprintf(stderr, "[fock-scf] iter %d kept kets per class %llu %llu %llu %llu %llu %llu\n",
+                                     iter, kc[0], kc[1], kc[2], kc[3], kc[4], kc[5]);
                 }
             } else {
                 STCHK(enqueue_jk_incremental(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau, tau64,
