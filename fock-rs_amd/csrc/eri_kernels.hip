@@ -520,6 +520,24 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     const int n = args.nbf;  // LDS is sized for the largest basis of the launch
     // the class's Boys slab: static LDS ahead of the dynamic part (32-bit LDS addressing)
     __shared__ Real slice[SLAB ? GN * COLS : 1];  // [GN][COLS]
+    // FP32: the slab's loads are issued here, into registers (7 per thread), and stored to LDS after
+    // the ket screen, so their latency overlaps the task and screen loads
+    constexpr bool SLAB_EARLY = SLAB && sizeof(Real) == 4;
+    constexpr int SLAB_PER = SLAB_EARLY ? (GN * COLS + 255) / 256 : 1;
+    const Real *slab_src = nullptr;
+    if constexpr (SLAB) {
+        if constexpr (sizeof(Real) == 8) slab_src = reinterpret_cast<const Real *>(args.bt.pd64);
+        else slab_src = reinterpret_cast<const Real *>(args.bt.pd32);
+        slab_src += (size_t)L * GN * COLS;
+    }
+    Real slab_reg[SLAB_PER];
+    if constexpr (SLAB_EARLY) {
+#pragma unroll
+        for (int j = 0; j < SLAB_PER; ++j) {
+            const int e = (int)threadIdx.x + 256 * j;
+            slab_reg[j] = e < GN * COLS ? slab_src[e] : Real(0);
+        }
+    }
     double *lds = reinterpret_cast<double *>(lds_bytes);
     double *Ka = lds;                 // [NA][n]
     double *Kb = lds + NA * n;        // [NB][n]
@@ -535,6 +553,13 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
     const PairInfo bra = args.pairs[blk.bra];
     const size_t N = (size_t)n;
     const double *__restrict__ P = args.P;
+    const int fa = bra.fa, fb = bra.fb;
+    // the bra's density block, loaded as soon as the bra is known (used from the first ket on)
+    Real pab[NAB];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)P[(fa + a) * N + (fb + b)];
     if (threadIdx.x < blk.ntasks) {
         const Task tk = args.tasks[blk.first_task + threadIdx.x];
         s_task[threadIdx.x] = tk;
@@ -588,28 +613,23 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         }
     }
     for (int e = threadIdx.x; e < (NA + NB) * n; e += blockDim.x) lds[e] = 0.0;
-    {
-        if constexpr (SLAB) {
-            const Real *slab;
-            if constexpr (sizeof(Real) == 8) slab = reinterpret_cast<const Real *>(args.bt.pd64);
-            else slab = reinterpret_cast<const Real *>(args.bt.pd32);
-            slab += (size_t)L * GN * COLS;
-            for (int e = threadIdx.x; e < GN * COLS; e += blockDim.x) slice[e] = slab[e];
+    if constexpr (SLAB_EARLY) {
+#pragma unroll
+        for (int j = 0; j < SLAB_PER; ++j) {
+            const int e = (int)threadIdx.x + 256 * j;
+            if (e < GN * COLS) slice[e] = slab_reg[j];
         }
+    } else if constexpr (SLAB) {
+        for (int e = threadIdx.x; e < GN * COLS; e += blockDim.x) slice[e] = slab_src[e];
     }
     __syncthreads();
-    const int fa = bra.fa, fb = bra.fb;
     double *__restrict__ J = args.J;
-    Real pab[NAB];
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) pab[a * NB + b] = (Real)P[(fa + a) * N + (fb + b)];
     double jab[NAB];
 #pragma unroll
     for (int x = 0; x < NAB; ++x) jab[x] = 0.0;
 
     // groups of 64 kept kets, one lane each
+    constexpr bool P_EARLY = NCD + NB * ND + NB * NC + NA * ND + NA * NC <= 16;
     for (int g = wave * 64; g < kept; g += 256) {
         const bool active = g + lane < kept;
         const int slot = active ? g + lane : g;
@@ -618,10 +638,54 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
         const int ket_idx = s_kidx[slot];
         const PairInfo ket = args.soa_pairs[ket_idx];
         const int ket_stride = args.soa_stride[ket_idx];
+        const int fc = ket.fa, fd = ket.fb;
+        // the density elements the digestion reads: for the small classes loaded before the
+        // integrals (their latency hides behind the primitive loop), else where they are used
+        Real p_cd[P_EARLY ? NCD : 1], p_bd[P_EARLY ? NB * ND : 1], p_bc[P_EARLY ? NB * NC : 1];
+        Real p_ad[P_EARLY ? NA * ND : 1], p_ac[P_EARLY ? NA * NC : 1];
+        if constexpr (P_EARLY) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) p_cd[c * ND + d] = (Real)P[(fc + c) * N + (fd + d)];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) p_bd[b * ND + d] = (Real)P[(fb + b) * N + (fd + d)];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) p_bc[b * NC + c] = (Real)P[(fb + b) * N + (fc + c)];
+            }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+#pragma unroll
+                for (int d = 0; d < ND; ++d) p_ad[a * ND + d] = (Real)P[(fa + a) * N + (fd + d)];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) p_ac[a * NC + c] = (Real)P[(fa + a) * N + (fc + c)];
+            }
+        }
+        auto Pcd = [&](int c, int d) -> Real {
+            if constexpr (P_EARLY) return p_cd[c * ND + d];
+            else return (Real)P[(fc + c) * N + (fd + d)];
+        };
+        auto Pbd = [&](int b, int d) -> Real {
+            if constexpr (P_EARLY) return p_bd[b * ND + d];
+            else return (Real)P[(fb + b) * N + (fd + d)];
+        };
+        auto Pbc = [&](int b, int c) -> Real {
+            if constexpr (P_EARLY) return p_bc[b * NC + c];
+            else return (Real)P[(fb + b) * N + (fc + c)];
+        };
+        auto Pad = [&](int a, int d) -> Real {
+            if constexpr (P_EARLY) return p_ad[a * ND + d];
+            else return (Real)P[(fa + a) * N + (fd + d)];
+        };
+        auto Pac = [&](int a, int c) -> Real {
+            if constexpr (P_EARLY) return p_ac[a * NC + c];
+            else return (Real)P[(fa + a) * N + (fc + c)];
+        };
         Real I[NAB][NCD];
         lane_integrals_lds<LA, LB, LC, LD, POLICY, REFSCREEN, Real, SLAB>(args, bra, ket, ket_stride, active,
                                                                           (LdsPtr<Real>)slice, I);
-        const int fc = ket.fa, fd = ket.fb;
         Real w = Real(1);
         if (fa == fb) w *= Real(0.5);
         if (fc == fd) w *= Real(0.5);
@@ -635,7 +699,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fc + c) * N + (fd + d)];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * Pcd(c, d);
                 jab[a * NB + b] += (double)(s * w);
             }
         if (!active) continue;
@@ -658,7 +722,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fd + d)];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * Pbd(b, d);
                 atomicAdd(&Ka[a * n + fc + c], (double)(s * w));
             }
 #pragma unroll
@@ -667,7 +731,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fb + b) * N + (fc + c)];
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * Pbc(b, c);
                 atomicAdd(&Ka[a * n + fd + d], (double)(s * w));
             }
         }
@@ -679,7 +743,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
 #pragma unroll
                 for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fd + d)];
+                    for (int d = 0; d < ND; ++d) s += I[a * NB + b][c * ND + d] * Pad(a, d);
                 atomicAdd(&Kb[b * n + fc + c], (double)(s * w));
             }
 #pragma unroll
@@ -688,7 +752,7 @@ __global__ __launch_bounds__(256) void eri_block_kernel(JKArgs args_in) {
 #pragma unroll
                 for (int a = 0; a < NA; ++a)
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * (Real)P[(fa + a) * N + (fc + c)];
+                    for (int c = 0; c < NC; ++c) s += I[a * NB + b][c * ND + d] * Pac(a, c);
                 atomicAdd(&Kb[b * n + fd + d], (double)(s * w));
             }
         }
