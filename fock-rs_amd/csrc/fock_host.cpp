@@ -92,6 +92,10 @@ struct DevBuf {
     }
     template <typename T>
     T *as() const { return reinterpret_cast<T *>(p); }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 struct Shell {
@@ -115,6 +119,7 @@ struct Prepared {
     DevBuf d_soa64, d_soa32, d_soa_pairs, d_soa_stride;  // ket-side primitive data per bucket (SoA)
     DevBuf d_Jk;  // ket-side J of the blocked kernel, [9][npair], all-zero between builds
     DevBuf d_prim_ab;  // exponents of both primitives of every primitive pair (gradients)
+    DevBuf d_prims64_tmp, d_prim_ab_tmp, d_soa_prim;  // preparation scratch (q-order permutation, SoA map)
     bool prim_screen = false;  // primitive pairs sorted by q and screened (accurate policy)
     bool prim_counted = false;  // stats.primitive_quartets holds the kept (screened) count
     int task_off[NUM_CLASSES + 1] = {0};
@@ -761,9 +766,13 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     mark("pairs_prims");
     const int npair = (int)pairs.size();
     const size_t nprim = prims.size();
+    // the one upload of the primitive pairs and their exponents: the q order, the FP32 copy and
+    // the ket-side SoA are made on the device from it
     HIPCHK(pr.d_prims64.ensure(nprim * sizeof(PrimPair<double>)));
     HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), nprim * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
                           S(ctx)));
+    HIPCHK(pr.d_prim_ab.ensure(std::max<size_t>(nprim, 1) * sizeof(double2)));
+    HIPCHK(hipMemcpyAsync(pr.d_prim_ab.p, prim_ab.data(), nprim * sizeof(double2), hipMemcpyHostToDevice, S(ctx)));
     pr.prim_screen = policy == FOCK_POLICY_ACCURATE;
     std::vector<float> prim_q_host(nprim, 0.0f);
     if (pr.prim_screen) {
@@ -789,9 +798,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         std::vector<float> q(nprim);
         HIPCHK(hipMemcpyAsync(q.data(), pr.d_prim_q.p, nprim * sizeof(float), hipMemcpyDeviceToHost, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
-        std::vector<PrimPair<double>> sorted(nprim);
-        std::vector<double2> sorted_ab(nprim);
-        std::vector<float> qs(nprim);
+        // src[e]: the primitive pair that goes to slot e (each shell pair's own range, stable)
+        std::vector<int> src(nprim);
         parallel_for((int)pairs.size(), 256, [&](int lo, int hi) {
             std::vector<int> perm;
             for (int t = lo; t < hi; ++t) {
@@ -801,38 +809,29 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
                 std::stable_sort(perm.begin(), perm.end(),
                                  [&](int x, int y) { return q[ph.pp_start + x] > q[ph.pp_start + y]; });
                 for (int i = 0; i < ph.K; ++i) {
-                    sorted[ph.pp_start + i] = prims[ph.pp_start + perm[i]];
-                    sorted_ab[ph.pp_start + i] = prim_ab[ph.pp_start + perm[i]];
-                    qs[ph.pp_start + i] = q[ph.pp_start + perm[i]];
+                    src[ph.pp_start + i] = ph.pp_start + perm[i];
+                    prim_q_host[ph.pp_start + i] = q[ph.pp_start + perm[i]];
                 }
             }
         });
-        prims.swap(sorted);
-        prim_ab.swap(sorted_ab);
-        prim_q_host = qs;
-        HIPCHK(hipMemcpyAsync(pr.d_prims64.p, prims.data(), nprim * sizeof(PrimPair<double>), hipMemcpyHostToDevice,
-                              S(ctx)));
-        HIPCHK(hipMemcpyAsync(pr.d_prim_q.p, qs.data(), nprim * sizeof(float), hipMemcpyHostToDevice, S(ctx)));
-        HIPCHK(hipStreamSynchronize(S(ctx)));  // host vectors go out of scope
+        HIPCHK(pr.d_prim_idx.ensure(std::max<size_t>(nprim, 1) * sizeof(int)));
+        HIPCHK(hipMemcpyAsync(pr.d_prim_idx.p, src.data(), nprim * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_prim_q.p, prim_q_host.data(), nprim * sizeof(float), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(pr.d_prims64_tmp.ensure(nprim * sizeof(PrimPair<double>)));
+        HIPCHK(pr.d_prim_ab_tmp.ensure(std::max<size_t>(nprim, 1) * sizeof(double2)));
+        HIPCHK(permute_prims((int64_t)nprim, pr.d_prim_idx.as<int>(), pr.d_prims64.as<PrimPair<double>>(),
+                             pr.d_prim_ab.as<double2>(), pr.d_prims64_tmp.as<PrimPair<double>>(),
+                             pr.d_prim_ab_tmp.as<double2>(), S(ctx)));
+        pr.d_prims64.swap(pr.d_prims64_tmp);
+        pr.d_prim_ab.swap(pr.d_prim_ab_tmp);
+        HIPCHK(hipStreamSynchronize(S(ctx)));  // src goes out of scope
     }
     if (precision == 1) {
-        std::vector<PrimPair<float>> p32(prims.size());
-        for (size_t i = 0; i < prims.size(); ++i) {
-            const PrimPair<double> &d = prims[i];
-            p32[i] = PrimPair<float>{(float)d.p,   (float)d.Px,  (float)d.Py,  (float)d.Pz,
-                                     (float)d.Kc,  (float)d.xax, (float)d.xay, (float)d.xaz,
-                                     (float)d.xbx, (float)d.xby, (float)d.xbz, (float)d.inv2p};
-        }
-        HIPCHK(pr.d_prims32.ensure(p32.size() * sizeof(PrimPair<float>)));
-        HIPCHK(hipMemcpyAsync(pr.d_prims32.p, p32.data(), p32.size() * sizeof(PrimPair<float>), hipMemcpyHostToDevice,
-                              S(ctx)));
-        HIPCHK(hipStreamSynchronize(S(ctx)));
+        HIPCHK(pr.d_prims32.ensure(nprim * sizeof(PrimPair<float>)));
+        HIPCHK(prims_to_f32((int64_t)nprim, pr.d_prims64.as<PrimPair<double>>(), pr.d_prims32.as<PrimPair<float>>(),
+                            S(ctx)));
     }
-
     mark("prim_screen");
-    HIPCHK(pr.d_prim_ab.ensure(std::max<size_t>(nprim, 1) * sizeof(double2)));
-    HIPCHK(hipMemcpyAsync(pr.d_prim_ab.p, prim_ab.data(), nprim * sizeof(double2), hipMemcpyHostToDevice, S(ctx)));
-    HIPCHK(hipStreamSynchronize(S(ctx)));  // prim_ab is a host temporary
 
     // diagonal (ab|ab): pairs grouped by class for the class-templated kernel
     std::vector<int> by_class(npair);
@@ -921,57 +920,40 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     {
         // ket-side primitive data, structure of arrays per bucket: field f of primitive k of the
         // bucket's pair j at base + (k * PRIM_FIELDS + f) * nb + (j - begin), so the 64 lanes of a
-        // task (consecutive pairs of one bucket) read each field with one contiguous access
-        std::vector<double> soa64((size_t)nprim * PRIM_FIELDS);
+        // task (consecutive pairs of one bucket) read each field with one contiguous access; laid
+        // out on the device from the (q-ordered) primitive pairs: per pair its column start and
+        // bucket size, per primitive pair its sorted pair and rank
         std::vector<PairInfo> sinfo(npair);
         std::vector<int> stride(npair);
-        const std::vector<float> &qh = prim_q_host;
-        const std::vector<PrimPair<double>> &ph = prims;
-        std::vector<size_t> bucket_base(buckets.size());
-        std::vector<int> bucket_of(npair);
-        {
-            size_t base = 0;
-            for (size_t bi = 0; bi < buckets.size(); ++bi) {
-                const Bucket &bk = buckets[bi];
-                bucket_base[bi] = base;
-                for (int j = bk.begin; j < bk.end; ++j) bucket_of[j] = (int)bi;
-                base += (size_t)(bk.end - bk.begin) * bk.K * PRIM_FIELDS;
-            }
-        }
-        parallel_for(npair, 256, [&](int lo, int hi) {  // every pair writes its own column of its bucket
-            for (int j = lo; j < hi; ++j) {
-                const Bucket &bk = buckets[bucket_of[j]];
-                const size_t base = bucket_base[bucket_of[j]];
-                const int nb = bk.end - bk.begin;
+        std::vector<int> prim_map(2 * nprim);  // [pair of e][rank k of e in its pair]
+        size_t base = 0;
+        for (const Bucket &bk : buckets) {
+            const int nb = bk.end - bk.begin;
+            for (int j = bk.begin; j < bk.end; ++j) {
                 const PairHost &pj = pr.pairs[j];
                 sinfo[j] = PairInfo{(int)(base + (j - bk.begin)), pj.K, pj.fa, pj.fb};
                 stride[j] = nb;
                 for (int k = 0; k < pj.K; ++k) {
-                    const PrimPair<double> &q = ph[pj.pp_start + k];
-                    const double f[PRIM_FIELDS] = {q.p,   q.Px,  q.Py,  q.Pz,  q.Kc,  q.xax,   q.xay,
-                                                   q.xaz, q.xbx, q.xby, q.xbz, q.inv2p, (double)qh[pj.pp_start + k]};
-                    for (int x = 0; x < PRIM_FIELDS; ++x)
-                        soa64[base + ((size_t)k * PRIM_FIELDS + x) * nb + (j - bk.begin)] = f[x];
+                    prim_map[pj.pp_start + k] = j;
+                    prim_map[nprim + pj.pp_start + k] = k;
                 }
             }
-        });
-        HIPCHK(pr.d_soa64.ensure(std::max<size_t>(soa64.size(), 1) * sizeof(double)));
-        HIPCHK(hipMemcpyAsync(pr.d_soa64.p, soa64.data(), soa64.size() * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
-        if (precision == 1) {
-            std::vector<float> soa32(soa64.size());
-            parallel_for((int)((soa64.size() + 4095) / 4096), 4, [&](int lo, int hi) {
-                const size_t e1 = std::min(soa64.size(), (size_t)hi * 4096);
-                for (size_t e = (size_t)lo * 4096; e < e1; ++e) soa32[e] = (float)soa64[e];
-            });
-            HIPCHK(pr.d_soa32.ensure(std::max<size_t>(soa32.size(), 1) * sizeof(float)));
-            HIPCHK(hipMemcpyAsync(pr.d_soa32.p, soa32.data(), soa32.size() * sizeof(float), hipMemcpyHostToDevice,
-                                  S(ctx)));
-            HIPCHK(hipStreamSynchronize(S(ctx)));  // soa32 goes out of scope
+            base += (size_t)nb * bk.K * PRIM_FIELDS;
         }
+        const size_t soa_len = base;  // = nprim * PRIM_FIELDS
         HIPCHK(pr.d_soa_pairs.ensure(std::max(npair, 1) * sizeof(PairInfo)));
         HIPCHK(pr.d_soa_stride.ensure(std::max(npair, 1) * sizeof(int)));
+        HIPCHK(pr.d_soa_prim.ensure(std::max<size_t>(prim_map.size(), 1) * sizeof(int)));
         HIPCHK(hipMemcpyAsync(pr.d_soa_pairs.p, sinfo.data(), npair * sizeof(PairInfo), hipMemcpyHostToDevice, S(ctx)));
         HIPCHK(hipMemcpyAsync(pr.d_soa_stride.p, stride.data(), npair * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pr.d_soa_prim.p, prim_map.data(), prim_map.size() * sizeof(int), hipMemcpyHostToDevice,
+                              S(ctx)));
+        HIPCHK(pr.d_soa64.ensure(std::max<size_t>(soa_len, 1) * sizeof(double)));
+        if (precision == 1) HIPCHK(pr.d_soa32.ensure(std::max<size_t>(soa_len, 1) * sizeof(float)));
+        HIPCHK(build_soa((int64_t)nprim, pr.d_soa_prim.as<int>(), pr.d_soa_prim.as<int>() + nprim,
+                         pr.d_soa_pairs.as<PairInfo>(), pr.d_soa_stride.as<int>(), pr.d_prims64.as<PrimPair<double>>(),
+                         pr.prim_screen ? pr.d_prim_q.as<float>() : nullptr, pr.d_soa64.as<double>(),
+                         precision == 1 ? pr.d_soa32.as<float>() : nullptr, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));  // host arrays go out of scope
     }
     auto keep = [&](double di, double dj) -> bool {

@@ -149,6 +149,14 @@ struct CompactBuffers {
     void *tmp;
     size_t tmp_bytes;
 };
+// preparation (prepare.hip): primitive pairs reordered by the host's permutation src (with their
+// exponent pairs), converted to FP32, and laid out as the ket-side bucket SoA
+hipError_t permute_prims(int64_t n, const int *src, const PrimPair<double> *in, const double2 *ab_in,
+                         PrimPair<double> *out, double2 *ab_out, hipStream_t s);
+hipError_t prims_to_f32(int64_t n, const PrimPair<double> *in, PrimPair<float> *out, hipStream_t s);
+hipError_t build_soa(int64_t n, const int *prim_pair, const int *prim_k, const PairInfo *sinfo, const int *stride,
+                     const PrimPair<double> *prims, const float *q, double *soa64, float *soa32, hipStream_t s);
+
 hipError_t screen_flags(const ScreenArgs &a, hipStream_t s);
 size_t compact_temp_bytes(int64_t slots);
 hipError_t compact_tasks(const Task *tasks, int ntasks, const int *gfirst, const int *glast, const int *flag,

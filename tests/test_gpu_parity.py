@@ -153,6 +153,31 @@ def test_jk_o108(ctx, o108, policy, screen, key, tol):
         assert st["near_threshold"] == 0
 
 
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
+def test_jk_concurrent_classes_match_serial(ctx, o108, precision):
+    """A blocked build runs its six class kernels on four streams at once; with per-class timing
+    on they run one after another on one stream. Both sum the same quartet contributions, in
+    another order of FP64 atomics: J and K agree to rounding, and the FP64 form matches the
+    oracle."""
+    say("jk o108 concurrent vs serial classes, precision %d" % precision)
+    fb, _ = m.flatten(m.octane_631g())
+    ctx.jk_prepare(fb, abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ, 1e-12, precision)
+    Jc, Kc = ctx.jk_build(o108["P_random"])
+    ctx.jk_set_timing(True)
+    try:
+        Js, Ks = ctx.jk_build(o108["P_random"])
+        ms = ctx.jk_timings()
+    finally:
+        ctx.jk_set_timing(False)
+    Jc2, Kc2 = ctx.jk_build(o108["P_random"])
+    assert (np.asarray(ms) > 0).all()
+    for A, B in ((Jc, Js), (Kc, Ks), (Jc, Jc2), (Kc, Kc2)):
+        assert np.abs(A - B).max() < 1e-11, np.abs(A - B).max()
+    if precision == abi.PRECISION_FP64:
+        assert np.abs(Jc - o108["o108_acc_J"]).max() < 1e-9
+        assert np.abs(Kc - o108["o108_acc_K"]).max() < 1e-9
+
+
 @pytest.mark.parametrize("policy,screen", [(abi.POLICY_ACCURATE, abi.SCREEN_SCHWARZ),
                                            (abi.POLICY_REF_COMPAT, abi.SCREEN_REF_BF_PAIR)])
 def test_screen_lists_are_bit_exact_between_runs(ctx, policy, screen):
