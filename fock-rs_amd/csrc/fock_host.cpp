@@ -3320,12 +3320,24 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             e_nuc += nuclei->charges[i] * nuclei->charges[j] / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         }
 
-    // direct J/K state (screen lists, pair data)
+    // direct J/K state (screen lists, pair data): prepared on a host thread of its own while this
+    // thread enqueues the one-electron integrals, X and the guess, which do not read it; joined
+    // before the first Fock build (and on every early return)
     const bool mixed = opts.precision == FOCK_PRECISION_MIXED;
     phase_mark("setup");
-    STCHK(fock_jk_prepare(ctx, basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0));
+    std::vector<Shell> shells_1e;
+    STCHK(build_shells(ctx, basis, shells_1e));
+    struct Preparation {
+        std::thread t;
+        fock_status status = FOCK_OK;
+        ~Preparation() {
+            if (t.joinable()) t.join();
+        }
+    } prep;
+    prep.t = std::thread([&] {
+        prep.status = fock_jk_prepare(ctx, basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0);
+    });
     Prepared &pr = ctx->jk;
-    phase_mark("jk_prepare");
 
     // device matrices
     DevBuf dS, dT, dV, dH, dX, dU, dTmp, dTmp2, dP, dG, dF, dFx, dC, dW, dE, dWork, dInfo, dErrs, dFocks, dDots,
@@ -3360,7 +3372,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     // S, T, V on the GPU (scf/src/lib.rs:118-129), H = T + V (:138)
     {
         BasisDev bd;
-        STCHK(upload_one_electron(ctx, basis, nuclei, pr.shells, bd));
+        STCHK(upload_one_electron(ctx, basis, nuclei, shells_1e, bd));
         STCHK(enqueue_one_electron(ctx, basis, nuclei, bd, opts.integral_policy, S_, dT.as<double>(), dV.as<double>()));
         HIPCHK(add_f64(nn, dT.as<double>(), dV.as<double>(), H_, S(ctx)));
         HIPCHK(hipStreamSynchronize(S(ctx)));
@@ -3509,6 +3521,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     }
 
     phase_mark("guess");
+    prep.t.join();
+    STCHK(prep.status);
+    phase_mark("jk_prepare");
     if (opts.autotune) {  // tiles for this molecule at the guess density, kept in the context
         fock_tune_report rep{};
         STCHK(tune_prepared(ctx, P_, &rep));
