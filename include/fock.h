@@ -173,7 +173,9 @@ void fock_ctx_destroy(fock_ctx *ctx);
 /* Message of the last failing call on this context ("" if none). */
 const char *fock_ctx_last_error(const fock_ctx *ctx);
 /* Launch on a caller-provided stream (a hipStream_t cast to void*); NULL restores the
- * context's own stream. */
+ * context's own stream. Every call's work is ordered on that stream: a blocked J/K build forks
+ * its class kernels onto three internal side streams (default priority) with events recorded on
+ * it and joins them back before it returns, so the caller sees one stream. */
 fock_status fock_ctx_set_stream(fock_ctx *ctx, void *hip_stream);
 /* Library version string. */
 const char *fock_version(void);
@@ -276,7 +278,8 @@ fock_status fock_sorted_eigs_batch(fock_ctx *ctx, int32_t count, int32_t n, cons
 fock_status fock_diis(fock_ctx *ctx, int32_t m, int64_t len, const double *errors, const double *focks, double *out);
 
 /* Per-class kernel timing of J/K builds: when enabled, every class launch is bracketed by HIP
- * events recorded on the launch stream. fock_jk_get_timings returns the milliseconds of each of
+ * events recorded on the launch stream, and the classes run one after another on that stream
+ * (otherwise they run concurrently). fock_jk_get_timings returns the milliseconds of each of
  * the six class kernels of the most recent build (waits for that build to finish). */
 fock_status fock_jk_set_timing(fock_ctx *ctx, int32_t enable);
 fock_status fock_jk_get_timings(fock_ctx *ctx, double ms[6]);
