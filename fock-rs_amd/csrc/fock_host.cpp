@@ -125,6 +125,11 @@ struct Prepared {
     int task_off[NUM_CLASSES + 1] = {0};
     int block_off[NUM_CLASSES + 1] = {0};
     std::vector<Task> h_tasks;    // the task lists of d_tasks (block lists are rebuilt from them)
+    // host arrays of the preparation, kept so that a preparation of the same size writes into
+    // memory that is already mapped (fresh pages cost a fault each)
+    std::vector<int> h_gfirst, h_glast;
+    std::vector<PrimPair<double>> h_prims;
+    std::vector<double2> h_prim_ab;
     std::vector<Block> h_blocks;  // the block lists of d_blocks
     int tile[NUM_CLASSES] = {DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE};
     fock_jk_stats stats{};
@@ -686,8 +691,10 @@ fock_status upload_blocks(fock_ctx *ctx, Prepared &pr) {
     // largest Q = sqrt(dmax) of every block's kets: the first ket of each task (kets of a task are
     // consecutive in descending dmax)
     std::vector<float> qmax(pr.h_blocks.size());
-    for (int cls = 0; cls < NUM_CLASSES; ++cls)
-        for (int b = pr.block_off[cls]; b < pr.block_off[cls + 1]; ++b) {
+    parallel_for((int)pr.h_blocks.size(), 2048, [&](int lo, int hi) {
+        int cls = 0;
+        for (int b = lo; b < hi; ++b) {
+            while (b >= pr.block_off[cls + 1]) ++cls;
             const Block &bk = pr.h_blocks[b];
             float m = 0.0f;
             for (int t = 0; t < bk.ntasks; ++t) {
@@ -696,6 +703,7 @@ fock_status upload_blocks(fock_ctx *ctx, Prepared &pr) {
             }
             qmax[b] = m;
         }
+    });
     HIPCHK(pr.d_block_qmax.ensure(std::max<size_t>(pr.h_tasks.size(), 1) * sizeof(float)));
     HIPCHK(hipMemcpyAsync(pr.d_block_qmax.p, qmax.data(), qmax.size() * sizeof(float), hipMemcpyHostToDevice,
                           S(ctx)));
@@ -749,8 +757,10 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
             pairs.push_back(ph);
         }
     // primitive pairs of every shell pair (a-major), filled in parallel at their offsets
-    std::vector<PrimPair<double>> prims(nprim_all);
-    std::vector<double2> prim_ab(nprim_all);  // exponents of the two primitives of every primitive pair
+    std::vector<PrimPair<double>> &prims = pr.h_prims;
+    std::vector<double2> &prim_ab = pr.h_prim_ab;  // exponents of the two primitives of every primitive pair
+    prims.resize(nprim_all);
+    prim_ab.resize(nprim_all);
     parallel_for((int)pairs.size(), 256, [&](int lo, int hi) {
         for (int t = lo; t < hi; ++t) {
             const PairHost &ph = pairs[t];
@@ -869,27 +879,37 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     }
 
     mark("diag_D");
-    // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation)
-    std::vector<int> order(npair);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int x, int y) {
-        const PairHost &a = pairs[x], &b = pairs[y];
-        if (a.cls != b.cls) return a.cls < b.cls;
-        if (a.K != b.K) return a.K < b.K;
-        if (a.dmax != b.dmax) return a.dmax > b.dmax;
-        return a.creation < b.creation;
+    // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation). The
+    // keys are sorted by value (no indirection into the pair records), the two sorts side by side
+    struct SortKey {
+        int cls, K;
+        double dmax;
+        int creation, idx;
+    };
+    std::vector<SortKey> ko(npair), kr;
+    for (int i = 0; i < npair; ++i) ko[i] = SortKey{pairs[i].cls, pairs[i].K, pairs[i].dmax, pairs[i].creation, i};
+    kr = ko;
+    parallel_for(2, 1, [&](int lo, int hi) {
+        for (int which = lo; which < hi; ++which) {
+            if (which == 0)
+                std::sort(ko.begin(), ko.end(), [](const SortKey &a, const SortKey &b) {
+                    if (a.cls != b.cls) return a.cls < b.cls;
+                    if (a.K != b.K) return a.K < b.K;
+                    if (a.dmax != b.dmax) return a.dmax > b.dmax;
+                    return a.creation < b.creation;
+                });
+            else
+                std::sort(kr.begin(), kr.end(), [](const SortKey &a, const SortKey &b) {
+                    if (a.cls != b.cls) return a.cls < b.cls;
+                    if (a.dmax != b.dmax) return a.dmax > b.dmax;
+                    return a.creation < b.creation;
+                });
+        }
     });
-    std::vector<int> rank_in_class(npair);
-    {
-        std::vector<int> r(npair);
-        std::iota(r.begin(), r.end(), 0);
-        std::sort(r.begin(), r.end(), [&](int x, int y) {
-            const PairHost &a = pairs[x], &b = pairs[y];
-            if (a.cls != b.cls) return a.cls < b.cls;
-            if (a.dmax != b.dmax) return a.dmax > b.dmax;
-            return a.creation < b.creation;
-        });
-        for (int i = 0; i < npair; ++i) rank_in_class[r[i]] = i;
+    std::vector<int> order(npair), rank_in_class(npair);
+    for (int i = 0; i < npair; ++i) {
+        order[i] = ko[i].idx;
+        rank_in_class[kr[i].idx] = i;
     }
     pr.pairs.resize(npair);
     std::vector<PairInfo> info(npair);
@@ -962,8 +982,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         return di * dj > pr.dthr;
     };
     mark("soa");
-    std::vector<Task> tasks;
-    std::vector<int> gfirst_all, glast_all;
+    std::vector<Task> &tasks = pr.h_tasks;
+    std::vector<int> &gfirst_all = pr.h_gfirst, &glast_all = pr.h_glast;
     fock_jk_stats st{};
     st.n_basis = n;
     st.n_shells = ns;
@@ -1087,7 +1107,6 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
                 glast_all[at + q] = (q + 1 < m && pc.cgid[q] == pc.cgid[q + 1]) ? glast_all[at + q + 1] : local + q;
         }
     });
-    pr.h_tasks = std::move(tasks);
     mark("task_join");
     for (int cls = 0; cls < NUM_CLASSES; ++cls) pr.tile[cls] = ctx->tiles[precision][cls];
     STCHK(upload_blocks(ctx, pr));
