@@ -1540,6 +1540,12 @@ fock_status time_launch(fock_ctx *ctx, EventPair &ev, F launch, double &best) {
 // fastest tile is kept in the prepared block lists and in the context's table of the prepared
 // form (FP64 or mixed: the mixed form times its FP32 bulk kernels). The timing launches write
 // into the build's scratch J/K; the pair-indexed J accumulator is cleared afterwards.
+bool valid_tile(int t) {
+    for (int k = 0; k < NUM_TILE_CHOICES; ++k)
+        if (TILE_CHOICES[k] == t) return true;
+    return false;
+}
+
 fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep) {
     Prepared &pr = ctx->jk;
     const auto t0 = std::chrono::steady_clock::now();
@@ -1582,17 +1588,63 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
     }
     STCHK(upload_blocks(ctx, pr));
     HIPCHK(fill_f64((int64_t)9 * (int64_t)pr.pairs.size(), 0.0, pr.d_Jk.as<double>(), S(ctx)));
+    // refinement on whole builds: a build runs its classes concurrently, so the tile that is best
+    // alone need not be best beside the others. Each class, longest first, tries its neighbouring
+    // tiles with the whole build timed (best of three) and keeps the fastest.
+    {
+        double best_build = 0.0;
+        auto time_build = [&](double &ms) -> fock_status {
+            return time_launch(ctx, ev, [&]() -> fock_status { return enqueue_jk(ctx, dP, Jb, Kb, MODE_JK, nullptr); },
+                               ms);
+        };
+        STCHK(time_build(best_build));
+        int order[NUM_CLASSES];
+        std::iota(order, order + NUM_CLASSES, 0);
+        auto best_alone = [&](int c) {
+            double m = HUGE_VAL;
+            for (int k = 0; k < NUM_TILE_CHOICES; ++k)
+                if (rep->ms[c][k] >= 0.0) m = std::min(m, rep->ms[c][k]);
+            return m == HUGE_VAL ? 0.0 : m;
+        };
+        std::stable_sort(order, order + NUM_CLASSES, [&](int x, int y) { return best_alone(x) > best_alone(y); });
+        auto fits = [&](int cls, int t) {
+            return valid_tile(t) && block_lds_bytes(cls, n, pr.policy, fp32, t) <= ctx->max_lds;
+        };
+        // up to two passes; in a pass a class moves in one direction (halving or doubling its
+        // tile) while the whole build gets faster
+        for (int pass = 0; pass < 2; ++pass) {
+            bool moved = false;
+            for (int cls : order) {
+                if (pr.task_off[cls + 1] == pr.task_off[cls]) continue;
+                for (int dir : {2, -2}) {
+                    bool better = false;
+                    for (;;) {
+                        const int cur = pr.tile[cls], cand = dir > 0 ? cur * 2 : cur / 2;
+                        if (!fits(cls, cand)) break;
+                        pr.tile[cls] = cand;
+                        STCHK(upload_blocks(ctx, pr));
+                        double ms = 0.0;
+                        STCHK(time_build(ms));
+                        if (!(ms < best_build)) {
+                            pr.tile[cls] = cur;
+                            break;
+                        }
+                        best_build = ms;
+                        better = moved = true;
+                    }
+                    if (better) break;  // this class moved; the other direction is not tried
+                }
+            }
+            if (!moved) break;
+        }
+        STCHK(upload_blocks(ctx, pr));
+    }
     HIPCHK(hipStreamSynchronize(S(ctx)));
     for (int c = 0; c < NUM_CLASSES; ++c) ctx->tiles[pr.precision][c] = rep->tile[c] = pr.tile[c];
     rep->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FOCK_OK;
 }
 
-bool valid_tile(int t) {
-    for (int k = 0; k < NUM_TILE_CHOICES; ++k)
-        if (TILE_CHOICES[k] == t) return true;
-    return false;
-}
 }  // namespace
 
 extern "C" fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report) {

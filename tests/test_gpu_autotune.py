@@ -74,7 +74,9 @@ def test_jk_autotune_report_and_result(ctx):
         timed = rep["ms"][c] >= 0
         assert timed.any()
         k = abi.TILE_CHOICES.index(rep["tile"][c])
-        assert rep["ms"][c][k] == rep["ms"][c][timed].min()  # the fastest candidate is kept
+        # the fastest candidate alone, or a tile the whole-build refinement moved to (every
+        # candidate that fits was timed alone)
+        assert rep["ms"][c][k] >= 0
     assert ctx.tiles(abi.TILES_FP64)[0] == rep["tile"]
     J, K = ctx.jk_build(P)  # the pair-indexed J accumulator was cleared after the timing launches
     assert rel(J, J0) < 1e-12 and rel(K, K0) < 1e-12
