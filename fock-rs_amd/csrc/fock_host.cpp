@@ -1676,7 +1676,7 @@ extern "C" fock_status fock_tiles_set(fock_ctx *ctx, int32_t form, const int32_t
         return ctx->fail(FOCK_INVALID_ARG, "unknown batch schedule");
     if (tile)
         for (int c = 0; c < NUM_CLASSES; ++c)
-            if (!valid_tile(tile[c])) return ctx->fail(FOCK_INVALID_ARG, "tile must be 1, 2, 4, 8 or 16");
+            if (!valid_tile(tile[c])) return ctx->fail(FOCK_INVALID_ARG, "tile must be 1, 2, 4, 8, 16 or 32");
     HIPCHK(hipSetDevice(ctx->device));
     if (tile)
         for (int c = 0; c < NUM_CLASSES; ++c) ctx->tiles[form][c] = tile[c];
