@@ -1610,30 +1610,27 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
         auto fits = [&](int cls, int t) {
             return valid_tile(t) && block_lds_bytes(cls, n, pr.policy, fp32, t) <= ctx->max_lds;
         };
-        // up to two passes; in a pass a class moves in one direction (halving or doubling its
-        // tile) while the whole build gets faster
+        // up to two passes; in a pass each class tries every tile that fits with the others fixed
         for (int pass = 0; pass < 2; ++pass) {
             bool moved = false;
             for (int cls : order) {
                 if (pr.task_off[cls + 1] == pr.task_off[cls]) continue;
-                for (int dir : {2, -2}) {
-                    bool better = false;
-                    for (;;) {
-                        const int cur = pr.tile[cls], cand = dir > 0 ? cur * 2 : cur / 2;
-                        if (!fits(cls, cand)) break;
-                        pr.tile[cls] = cand;
-                        STCHK(upload_blocks(ctx, pr));
-                        double ms = 0.0;
-                        STCHK(time_build(ms));
-                        if (!(ms < best_build)) {
-                            pr.tile[cls] = cur;
-                            break;
-                        }
+                const int start = pr.tile[cls];
+                int keep = start;
+                for (int k = 0; k < NUM_TILE_CHOICES; ++k) {
+                    const int cand = TILE_CHOICES[k];
+                    if (cand == start || !fits(cls, cand)) continue;
+                    pr.tile[cls] = cand;
+                    STCHK(upload_blocks(ctx, pr));
+                    double ms = 0.0;
+                    STCHK(time_build(ms));
+                    if (ms < best_build) {
                         best_build = ms;
-                        better = moved = true;
+                        keep = cand;
                     }
-                    if (better) break;  // this class moved; the other direction is not tried
                 }
+                pr.tile[cls] = keep;
+                if (keep != start) moved = true;
             }
             if (!moved) break;
         }
@@ -1676,7 +1673,7 @@ extern "C" fock_status fock_tiles_set(fock_ctx *ctx, int32_t form, const int32_t
         return ctx->fail(FOCK_INVALID_ARG, "unknown batch schedule");
     if (tile)
         for (int c = 0; c < NUM_CLASSES; ++c)
-            if (!valid_tile(tile[c])) return ctx->fail(FOCK_INVALID_ARG, "tile must be 1, 2, 4, 8, 16 or 32");
+            if (!valid_tile(tile[c])) return ctx->fail(FOCK_INVALID_ARG, "tile must be 1, 2, 4, 8, 16, 32 or 64");
     HIPCHK(hipSetDevice(ctx->device));
     if (tile)
         for (int c = 0; c < NUM_CLASSES; ++c) ctx->tiles[form][c] = tile[c];

@@ -122,9 +122,9 @@ constexpr int MAX_PRIM_PAIRS = 36;
 // tasks of one bra pair per workgroup of the blocked digestion (one wave per task at a time):
 // the tile, chosen per quartet class from TILE_CHOICES (default DEFAULT_TILE) by the autotuner
 constexpr int DEFAULT_TILE = 8;
-constexpr int MAX_TILE = 32;
-constexpr int NUM_TILE_CHOICES = 6;
-constexpr int TILE_CHOICES[NUM_TILE_CHOICES] = {1, 2, 4, 8, 16, 32};
+constexpr int MAX_TILE = 64;
+constexpr int NUM_TILE_CHOICES = 7;
+constexpr int TILE_CHOICES[NUM_TILE_CHOICES] = {1, 2, 4, 8, 16, 32, 64};
 // largest basis for the blocked digestion: (3 + 3) K rows of n doubles in LDS next to the staged
 // Boys slab (at most 33 KB) and the block's ket records (0.3 KB per task of the tile)
 constexpr int BLOCKED_MAX_NBF = 600;

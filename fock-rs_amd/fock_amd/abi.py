@@ -37,7 +37,7 @@ TILES_MIXED = 1
 TILES_BATCH = 2
 SCHEDULE_COST = 0
 SCHEDULE_XCD = 1
-TILE_CHOICES = (1, 2, 4, 8, 16, 32)
+TILE_CHOICES = (1, 2, 4, 8, 16, 32, 64)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -99,7 +99,7 @@ class GridDesc(ctypes.Structure):
 
 
 class TuneReport(ctypes.Structure):
-    _fields_ = [("tile", ctypes.c_int32 * 6), ("ms", (ctypes.c_double * 6) * 6), ("schedule", ctypes.c_int32),
+    _fields_ = [("tile", ctypes.c_int32 * 6), ("ms", (ctypes.c_double * 7) * 6), ("schedule", ctypes.c_int32),
                 ("schedule_ms", ctypes.c_double * 2), ("tune_ms", ctypes.c_double)]
 
 

@@ -365,7 +365,7 @@ typedef enum fock_batch_schedule {
 } fock_batch_schedule;
 typedef struct fock_tune_report {
     int32_t tile[6];         /* chosen tile per quartet class */
-    double ms[6][6];         /* per class and candidate tile 1, 2, 4, 8, 16, 32: best of three
+    double ms[6][7];         /* per class and candidate tile 1, 2, 4, 8, 16, 32, 64: best of three
                               * launches in ms (batched: under the chosen schedule); -1 not timed */
     int32_t schedule;        /* batched builds: chosen fock_batch_schedule */
     double schedule_ms[2];   /* batched builds: sum over classes of the best time per schedule */
@@ -375,11 +375,12 @@ typedef struct fock_tune_report {
 /* Time every candidate tile of each class on the prepared J/K build (fock_jk_prepare) at density
  * P (host, row-major n x n), keep the fastest for the prepared build and for this context's later
  * builds of the prepared form (FP64 or mixed). The class kernels of a build run concurrently, so
- * the per-class choice is then refined on whole builds: each class, longest first, halves or
- * doubles its tile while the whole build (timed, best of three) gets faster, for up to two passes
- * (report->ms holds the per-class times, report->tile the final choice). report may be NULL. */
+ * the per-class choice is then refined on whole builds: each class, longest first, tries every
+ * tile with the others fixed and keeps the one with the fastest whole build (timed, best of
+ * three), for up to two passes (report->ms holds the per-class times, report->tile the final
+ * choice). report may be NULL. */
 fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report);
-/* Read or set the context's tiles of a form (fock_tile_form); tiles are 1, 2, 4, 8, 16 or 32 (default
+/* Read or set the context's tiles of a form (fock_tile_form); tiles are 1, 2, 4, 8, 16, 32 or 64 (default
  * 8). schedule applies to FOCK_TILES_BATCH (fock_batch_schedule; default COST). tile may be NULL. */
 fock_status fock_tiles_get(fock_ctx *ctx, int32_t form, int32_t tile[6], int32_t *schedule);
 fock_status fock_tiles_set(fock_ctx *ctx, int32_t form, const int32_t tile[6], int32_t schedule);

@@ -47,7 +47,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(abi.ScfOpts) == 120
     assert ctypes.sizeof(abi.ScfIter) == 32
     assert ctypes.sizeof(abi.JKStats) == 12 + 4 + 6 * 8 * 5 + 8 + 8 + 8
-    assert ctypes.sizeof(abi.TuneReport) == 6 * 4 + 6 * 6 * 8 + 4 + 4 + 2 * 8 + 8
+    assert ctypes.sizeof(abi.TuneReport) == 6 * 4 + 6 * 7 * 8 + 4 + 4 + 2 * 8 + 8
 
 
 def test_component_order_is_the_references():
