@@ -1326,15 +1326,21 @@ struct ClassStreams {
     int order(int k) const { return concurrent ? CLASS_LAUNCH_ORDER[k] : k; }
 };
 
-fock_status fork_classes(fock_ctx *ctx, ClassStreams &cs) {
-    cs.ctx = ctx;
-    cs.concurrent = !ctx->timing;
-    if (!cs.concurrent) return FOCK_OK;
+// the side streams and fork/join events (made once per context; before a stream capture)
+fock_status ensure_class_streams(fock_ctx *ctx) {
     for (int i = 0; i < CLASS_STREAMS - 1; ++i) {
         if (!ctx->side[i]) HIPCHK(hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
         if (!ctx->join_ev[i]) HIPCHK(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
     }
     if (!ctx->fork_ev) HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    return FOCK_OK;
+}
+
+fock_status fork_classes(fock_ctx *ctx, ClassStreams &cs) {
+    cs.ctx = ctx;
+    cs.concurrent = !ctx->timing;
+    if (!cs.concurrent) return FOCK_OK;
+    STCHK(ensure_class_streams(ctx));
     HIPCHK(hipEventRecord(ctx->fork_ev, S(ctx)));
     for (int i = 0; i < CLASS_STREAMS - 1; ++i) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0));
     return FOCK_OK;
@@ -2729,8 +2735,8 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         return FOCK_OK;
     };
     const bool refscreen = opts.screen == FOCK_SCREEN_REF_BF_PAIR;
-    // one class launch over the whole batch
-    auto launch_class = [&](int cls) -> fock_status {
+    // one class launch over the whole batch (on stream `on`, default the batch's stream)
+    auto launch_class = [&](int cls, hipStream_t on = nullptr) -> fock_status {
         if (bbl[cls].empty()) return FOCK_OK;
         JKArgs a{};
         a.nbf = nmax;
@@ -2741,7 +2747,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         a.prim_thr = PRIM_THRESHOLD;
         a.batch = d_recs.as<BatchRec>() + 0;
         a.bblocks = d_bbl[cls].as<int2>();
-        HIPCHK(launch_eri_blocks(cls, a, opts.integral_policy, refscreen, false, st));
+        HIPCHK(launch_eri_blocks(cls, a, opts.integral_policy, refscreen, false, on ? on : st));
         return FOCK_OK;
     };
     auto fits = [&](int cls, int tile) {
@@ -2999,7 +3005,15 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Jb.as<double>(), st));
             HIPCHK(fill_f64(nn, 0.0, bp->pr.d_Kb.as<double>(), st));
         }
-        for (int cls = 0; cls < NUM_CLASSES; ++cls) STCHK(launch_class(cls));
+        // the six class launches run concurrently, as in a one-molecule build (forked from and
+        // joined back into the batch's stream; under capture the fork and join are graph edges)
+        ClassStreams cs;
+        STCHK(fork_classes(ctx, cs));
+        for (int k = 0; k < NUM_CLASSES; ++k) {
+            const int cls = cs.order(k);
+            STCHK(launch_class(cls, cs.of(cls)));
+        }
+        STCHK(join_classes(ctx, cs));
         HIPCHK(scatter_jk_batch(count, d_scat.as<ScatterRec>(), max_npair, st));
         for (auto &bp : mols)
             HIPCHK(fock_from_jk(bp->n, bp->H.as<double>(), bp->pr.d_Jb.as<double>(), bp->pr.d_Kb.as<double>(),
@@ -3014,6 +3028,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
             if (g) (void)hipGraphDestroy(g);
         }
     } graph;
+    STCHK(ensure_class_streams(ctx));
     if (st != nullptr && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
         const fock_status cs = enqueue_build();
         hipGraph_t g = nullptr;
@@ -3026,6 +3041,7 @@ extern "C" fock_status fock_scf_batch_run(fock_ctx *ctx, int32_t count, const fo
         }
         (void)hipGetLastError();  // a failed capture leaves no launch behind; fall back to eager
     }
+    if (debug) std::fprintf(stderr, "[fock-batch] build graph %s\n", graph.x ? "captured" : "not captured (eager)");
     // per molecule m: dots_all[18 m + s] = <e, e_s> (s < 16), [18 m + 16] = max |e|, [18 m + 17] = E_elec
     DevBuf d_dots;
     HIPCHK(d_dots.ensure((size_t)count * 18 * sizeof(double)));
