@@ -3313,8 +3313,12 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
         g.grad = dG.as<double>();
         HIPCHK(launch_grad_one(c, g, S(ctx)));
     }
-    // two-electron part over the unique quartets of the Fock build
-    for (int cls = 0; cls < NUM_CLASSES; ++cls) {
+    // two-electron part over the unique quartets of the Fock build; the class kernels run
+    // concurrently (their contributions meet in the gradient's atomics)
+    ClassStreams cs;
+    STCHK(fork_classes(ctx, cs));
+    for (int k = 0; k < NUM_CLASSES; ++k) {
+        const int cls = cs.order(k);
         GradEriArgs g{};
         g.tasks = pr.d_tasks.as<Task>() + pr.task_off[cls];
         g.ntasks = pr.task_off[cls + 1] - pr.task_off[cls];
@@ -3337,8 +3341,9 @@ extern "C" fock_status fock_rhf_gradient(fock_ctx *ctx, const fock_basis_desc *b
             g.prim_q = pr.d_prim_q.as<float>();
             g.prim_thr = prim_thr_grad;
         }
-        HIPCHK(launch_grad_eri(cls, g, S(ctx)));
+        HIPCHK(launch_grad_eri(cls, g, cs.of(cls)));
     }
+    STCHK(join_classes(ctx, cs));
     std::vector<double> gh(3 * na);
     HIPCHK(hipMemcpyAsync(gh.data(), dG.p, 3 * na * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
     HIPCHK(hipStreamSynchronize(S(ctx)));
