@@ -358,7 +358,8 @@ fock_status build_shells(fock_ctx *ctx, const fock_basis_desc *b, std::vector<Sh
             for (int x = 0; x < 3; ++x)
                 if (!std::isfinite(b->centers[3 * g + x]))
                     return ctx->fail(FOCK_INVALID_ARG, "function " + std::to_string(g) + " has a non-finite centre");
-            if (std::memcmp(b->centers + 3 * g, b->centers + 3 * f, 3 * sizeof(double)) != 0)
+            if (b->centers[3 * g] != b->centers[3 * f] || b->centers[3 * g + 1] != b->centers[3 * f + 1] ||
+                b->centers[3 * g + 2] != b->centers[3 * f + 2])  // by value: +0.0 and -0.0 are one position
                 return ctx->fail(FOCK_INVALID_ARG, "shell components on different centres");
             for (int k = 0; k < K; ++k) {
                 const double e = b->exponents[b->prim_offset[g] + k], c = b->coefficients[b->prim_offset[g] + k];
