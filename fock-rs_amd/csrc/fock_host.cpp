@@ -64,10 +64,12 @@ constexpr float GRAD_TAU64 = 1e-4f;
 constexpr int CLASS_NINT[NUM_CLASSES] = {1, 3, 9, 9, 27, 81};
 // The class kernels of a blocked build are independent (their J/K contributions meet only in
 // atomics), so they run on CLASS_STREAMS streams at once: one kernel's tail and its idle issue
-// slots are filled by the others' workgroups. Stream of each class: the two largest classes alone,
-// the four smaller ones in pairs; the launch order puts the most unbalanced kernel, (pp|pp), first.
-constexpr int CLASS_STREAMS = 4;
-constexpr int CLASS_LANE[NUM_CLASSES] = {0, 1, 2, 3, 2, 3};
+// slots are filled by the other stream's workgroups. Two streams (measured: three or four, each on
+// a hardware queue of its own, were 5-25% slower: more classes at once share the same issue and
+// atomic throughput with more set-up in flight); the context's stream takes (pp|pp), (ss|ss) and
+// (pp|ss), the side stream (ps|ss), (ps|ps) and (pp|ps); (pp|pp), the most unbalanced kernel, first.
+constexpr int CLASS_STREAMS = 2;
+constexpr int CLASS_LANE[NUM_CLASSES] = {0, 1, 1, 0, 1, 0};
 constexpr int CLASS_LAUNCH_ORDER[NUM_CLASSES] = {5, 0, 1, 2, 3, 4};
 
 struct DevBuf {

@@ -174,7 +174,7 @@ void fock_ctx_destroy(fock_ctx *ctx);
 const char *fock_ctx_last_error(const fock_ctx *ctx);
 /* Launch on a caller-provided stream (a hipStream_t cast to void*); NULL restores the
  * context's own stream. Every call's work is ordered on that stream: a blocked J/K build forks
- * its class kernels onto three internal side streams (default priority) with events recorded on
+ * its class kernels onto an internal side stream (default priority) with events recorded on
  * it and joins them back before it returns, so the caller sees one stream. */
 fock_status fock_ctx_set_stream(fock_ctx *ctx, void *hip_stream);
 /* Library version string. */

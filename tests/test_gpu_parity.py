@@ -155,7 +155,7 @@ def test_jk_o108(ctx, o108, policy, screen, key, tol):
 
 @pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
 def test_jk_concurrent_classes_match_serial(ctx, o108, precision):
-    """A blocked build runs its six class kernels on four streams at once; with per-class timing
+    """A blocked build runs its six class kernels on two streams at once; with per-class timing
     on they run one after another on one stream. Both sum the same quartet contributions, in
     another order of FP64 atomics: J and K agree to rounding, and the FP64 form matches the
     oracle."""
