@@ -1525,14 +1525,14 @@ struct EventPair {
     }
 };
 
-// Best of three timed launches (after one untimed) of `launch` on the context's stream, in ms.
+// Best of `reps` (default three) timed launches (after one untimed) of `launch` on the context's stream, in ms.
 template <typename F>
-fock_status time_launch(fock_ctx *ctx, EventPair &ev, F launch, double &best) {
+fock_status time_launch(fock_ctx *ctx, EventPair &ev, F launch, double &best, int reps = 3) {
     if (!ev.a) HIPCHK(hipEventCreate(&ev.a));
     if (!ev.b) HIPCHK(hipEventCreate(&ev.b));
     STCHK(launch());
     best = HUGE_VAL;
-    for (int rep = 0; rep < 3; ++rep) {
+    for (int rep = 0; rep < reps; ++rep) {
         HIPCHK(hipEventRecord(ev.a, S(ctx)));
         STCHK(launch());
         HIPCHK(hipEventRecord(ev.b, S(ctx)));
@@ -1604,7 +1604,7 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
         double best_build = 0.0;
         auto time_build = [&](double &ms) -> fock_status {
             return time_launch(ctx, ev, [&]() -> fock_status { return enqueue_jk(ctx, dP, Jb, Kb, MODE_JK, nullptr); },
-                               ms);
+                               ms, 5);
         };
         STCHK(time_build(best_build));
         int order[NUM_CLASSES];

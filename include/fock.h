@@ -377,7 +377,7 @@ typedef struct fock_tune_report {
  * builds of the prepared form (FP64 or mixed). The class kernels of a build run concurrently, so
  * the per-class choice is then refined on whole builds: each class, longest first, tries every
  * tile with the others fixed and keeps the one with the fastest whole build (timed, best of
- * three), for up to two passes (report->ms holds the per-class times, report->tile the final
+ * five), for up to two passes (report->ms holds the per-class times, report->tile the final
  * choice). report may be NULL. */
 fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report);
 /* Read or set the context's tiles of a form (fock_tile_form); tiles are 1, 2, 4, 8, 16, 32 or 64 (default
