@@ -130,6 +130,18 @@ __device__ __forceinline__ PrimPair<Real> scalar_prim(ConstPtr<Real> base, int i
     return b;
 }
 
+// Issues the scalar loads of a bra primitive's q and record at this point: an empty asm that takes
+// them as operands, so that the compiler waits once for all of them before the screen test instead
+// of once for q and again for the record inside the branch that uses it (the loop's two scalar-load
+// waits per bra primitive were the FP32 primitive loop's longest stalls). LAB = 0 reads no Hermite
+// data.
+template <int LAB, typename Real>
+__device__ __forceinline__ void bra_loads_here(float q, const PrimPair<Real> &b) {
+    asm volatile("" ::"s"(q), "s"(b.p), "s"(b.Px), "s"(b.Py), "s"(b.Pz), "s"(b.Kc));
+    if constexpr (LAB > 0)
+        asm volatile("" ::"s"(b.xax), "s"(b.xay), "s"(b.xaz), "s"(b.xbx), "s"(b.xby), "s"(b.xbz), "s"(b.inv2p));
+}
+
 // Same integrals with the Boys slab in LDS, the ket primitive pairs read from the bucket-SoA copy
 // (the 64 lanes of a task read consecutive entries of every field: each ket primitive pair is
 // loaded once per lane and reused for every bra primitive), and the bra primitive pairs and their
@@ -168,10 +180,14 @@ __device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const Pai
                 }
                 if (__ballot(qk * q0 > thr) == 0) break;
                 for (int i = 0; i < bK; ++i) {
-                    const bool keep = cq[i] * qk > thr;
+                    // the record is read with its q (one scalar-load wait per bra primitive, not
+                    // one for q and another for the record after the test)
+                    const PrimPair<Real> bp = scalar_prim(cb, i);
+                    const float qi = cq[i];
+                    bra_loads_here<LA + LB>(qi, bp);
+                    const bool keep = qi * qk > thr;
                     if (__ballot(keep) == 0) break;
-                    if (keep)
-                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(scalar_prim(cb, i), kq, args.bt, I, slice);
+                    if (keep) accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(bp, kq, args.bt, I, slice);
                 }
             }
         } else {
@@ -180,10 +196,12 @@ __device__ __forceinline__ void lane_integrals_lds(const JKArgs &args, const Pai
                 if (__ballot(qk * q0 > thr) == 0) break;
                 const PrimPair<Real> kq = soa_prim(kbase, k, ket_stride);
                 for (int i = 0; i < bK; ++i) {
-                    const bool keep = cq[i] * qk > thr;
+                    const PrimPair<Real> bp = scalar_prim(cb, i);
+                    const float qi = cq[i];
+                    bra_loads_here<LA + LB>(qi, bp);
+                    const bool keep = qi * qk > thr;
                     if (__ballot(keep) == 0) break;
-                    if (keep)
-                        accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(scalar_prim(cb, i), kq, args.bt, I, slice);
+                    if (keep) accumulate_prim_quartet<LA, LB, LC, LD, POLICY, Real, SLAB>(bp, kq, args.bt, I, slice);
                 }
             }
         }
