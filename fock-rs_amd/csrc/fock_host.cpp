@@ -1598,15 +1598,40 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
     STCHK(upload_blocks(ctx, pr));
     HIPCHK(fill_f64((int64_t)9 * (int64_t)pr.pairs.size(), 0.0, pr.d_Jk.as<double>(), S(ctx)));
     // refinement on whole builds: a build runs its classes concurrently, so the tile that is best
-    // alone need not be best beside the others. Each class, longest first, tries its neighbouring
-    // tiles with the whole build timed (best of three) and keeps the fastest.
+    // alone need not be best beside the others. The search starts from the fastest whole build of
+    // three seeds (the per-class choice, every class at 32, every class at 64, where they fit),
+    // then each class, longest first, tries every tile with the others fixed and keeps the fastest
+    // (the whole build timed, best of five): a coordinate search over a flat, noisy landscape, so
+    // seeding it from large tiles keeps it from settling near a poor per-class start.
     {
+        auto fits = [&](int cls, int t) {
+            return valid_tile(t) && block_lds_bytes(cls, n, pr.policy, fp32, t) <= ctx->max_lds;
+        };
         double best_build = 0.0;
         auto time_build = [&](double &ms) -> fock_status {
+            STCHK(upload_blocks(ctx, pr));
             return time_launch(ctx, ev, [&]() -> fock_status { return enqueue_jk(ctx, dP, Jb, Kb, MODE_JK, nullptr); },
                                ms, 5);
         };
         STCHK(time_build(best_build));
+        {
+            int per_class[NUM_CLASSES], seed_best[NUM_CLASSES];
+            for (int c = 0; c < NUM_CLASSES; ++c) per_class[c] = seed_best[c] = pr.tile[c];
+            for (int seed : {32, 64}) {
+                for (int c = 0; c < NUM_CLASSES; ++c) {
+                    int t = seed;
+                    while (t > 1 && !fits(c, t)) t /= 2;
+                    pr.tile[c] = fits(c, t) ? t : per_class[c];
+                }
+                double ms = 0.0;
+                STCHK(time_build(ms));
+                if (ms < best_build) {
+                    best_build = ms;
+                    for (int c = 0; c < NUM_CLASSES; ++c) seed_best[c] = pr.tile[c];
+                }
+            }
+            for (int c = 0; c < NUM_CLASSES; ++c) pr.tile[c] = seed_best[c];
+        }
         int order[NUM_CLASSES];
         std::iota(order, order + NUM_CLASSES, 0);
         auto best_alone = [&](int c) {
@@ -1616,9 +1641,6 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
             return m == HUGE_VAL ? 0.0 : m;
         };
         std::stable_sort(order, order + NUM_CLASSES, [&](int x, int y) { return best_alone(x) > best_alone(y); });
-        auto fits = [&](int cls, int t) {
-            return valid_tile(t) && block_lds_bytes(cls, n, pr.policy, fp32, t) <= ctx->max_lds;
-        };
         // up to two passes; in a pass each class tries every tile that fits with the others fixed
         for (int pass = 0; pass < 2; ++pass) {
             bool moved = false;
@@ -1630,7 +1652,6 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
                     const int cand = TILE_CHOICES[k];
                     if (cand == start || !fits(cls, cand)) continue;
                     pr.tile[cls] = cand;
-                    STCHK(upload_blocks(ctx, pr));
                     double ms = 0.0;
                     STCHK(time_build(ms));
                     if (ms < best_build) {

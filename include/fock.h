@@ -375,9 +375,10 @@ typedef struct fock_tune_report {
 /* Time every candidate tile of each class on the prepared J/K build (fock_jk_prepare) at density
  * P (host, row-major n x n), keep the fastest for the prepared build and for this context's later
  * builds of the prepared form (FP64 or mixed). The class kernels of a build run concurrently, so
- * the per-class choice is then refined on whole builds: each class, longest first, tries every
- * tile with the others fixed and keeps the one with the fastest whole build (timed, best of
- * five), for up to two passes (report->ms holds the per-class times, report->tile the final
+ * the per-class choice is then refined on whole builds: starting from the fastest of the per-class
+ * choice, every class at 32 and every class at 64 (where they fit), each class, longest first,
+ * tries every tile with the others fixed and keeps the one with the fastest whole build (timed,
+ * best of five), for up to two passes (report->ms holds the per-class times, report->tile the final
  * choice). report may be NULL. */
 fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report);
 /* Read or set the context's tiles of a form (fock_tile_form); tiles are 1, 2, 4, 8, 16, 32 or 64 (default
