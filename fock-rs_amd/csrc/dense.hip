@@ -564,16 +564,67 @@ __device__ __forceinline__ void block_sum2(double &t, double &f) {
     f = rf[0];
 }
 
-// X = (hi I - A) / (hi - lo) with {lo, hi} = bounds (device); part[block] = (tr, sum of squares)
-// of the block's elements
+// Gershgorin interval of every row of a symmetric matrix: rowb[i] = {A_ii - r_i, A_ii + r_i},
+// r_i = sum_{j != i} |A_ij|; one wave per row, four rows per workgroup (the rows are read at once
+// by the whole device instead of one after another by one workgroup)
+__global__ __launch_bounds__(256) void gershgorin_rows_kernel(int n, const double *__restrict__ A,
+                                                              double *__restrict__ rowb) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;  // uniform per wave
+    const double *row = A + (size_t)i * n;
+    double r = 0.0;
+    for (int j = lane; j < n; j += 64)
+        if (j != i) r += fabs(row[j]);
+    r = wave_sum(r);
+    if (lane == 0) {
+        const double d = row[i];
+        rowb[2 * i] = d - r;
+        rowb[2 * i + 1] = d + r;
+    }
+}
+hipError_t gershgorin_rows(int n, const double *A, double *rowb, hipStream_t s) {
+    hipLaunchKernelGGL(gershgorin_rows_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, A, rowb);
+    return hipGetLastError();
+}
+
+// {lo, hi} = {min_i rowb[i].lo, max_i rowb[i].hi} over the n row intervals, to every thread of a
+// workgroup of 256
+__device__ __forceinline__ void rows_interval(int n, const double *__restrict__ rowb, double &lo, double &hi) {
+    __shared__ double sl[4], sh[4];
+    double l = HUGE_VAL, h = -HUGE_VAL;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        l = fmin(l, rowb[2 * i]);
+        h = fmax(h, rowb[2 * i + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l = fmin(l, __shfl_xor(l, o));
+        h = fmax(h, __shfl_xor(h, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sl[threadIdx.x >> 6] = l;
+        sh[threadIdx.x >> 6] = h;
+    }
+    __syncthreads();
+    lo = fmin(fmin(sl[0], sl[1]), fmin(sl[2], sl[3]));
+    hi = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// {lo, hi} from the row intervals (gershgorin_rows; workgroup 0 writes them to bounds), then
+// X = (hi I - A) / (hi - lo); part[block] = (tr, sum of squares) of the block's elements
 __global__ __launch_bounds__(256) void tc2_init_kernel(int n, const double *__restrict__ A,
-                                                       const double *__restrict__ bounds, double *__restrict__ X,
-                                                       double *__restrict__ part) {
+                                                       const double *__restrict__ rowb, double *__restrict__ bounds,
+                                                       double *__restrict__ X, double *__restrict__ part) {
+    double lo, hi;
+    rows_interval(n, rowb, lo, hi);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        bounds[0] = lo;
+        bounds[1] = hi;
+    }
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double t = 0.0, f = 0.0;
     if (e < (int64_t)n * n) {
         const int i = (int)(e / n), j = (int)(e % n);
-        const double lo = bounds[0], hi = bounds[1];
         const double v = ((i == j ? hi : 0.0) - A[e]) / (hi - lo);
         X[e] = v;
         t = i == j ? v : 0.0;
@@ -586,8 +637,10 @@ __global__ __launch_bounds__(256) void tc2_init_kernel(int n, const double *__re
     }
 }
 int tc2_init_parts(int n) { return (int)(((int64_t)n * n + 255) / 256); }
-hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, double *part, hipStream_t s) {
-    hipLaunchKernelGGL(tc2_init_kernel, dim3((unsigned)tc2_init_parts(n)), dim3(256), 0, s, n, A, bounds, X, part);
+hipError_t tc2_init(int n, const double *A, const double *rowb, double *bounds, double *X, double *part,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(tc2_init_kernel, dim3((unsigned)tc2_init_parts(n)), dim3(256), 0, s, n, A, rowb, bounds, X,
+                       part);
     return hipGetLastError();
 }
 

@@ -413,7 +413,10 @@ PrimPair<double> make_prim_pair(const fock_basis_desc *b, const Shell &sa, const
     return pp;
 }
 
-hipStream_t S(fock_ctx *ctx) { return ctx->stream; }
+// The stream a context's work is enqueued on: its own, or the stream this thread was given for
+// work that runs beside it (the SCF's preparation thread)
+thread_local hipStream_t tl_stream = nullptr;
+hipStream_t S(fock_ctx *ctx) { return tl_stream ? tl_stream : ctx->stream; }
 
 }  // namespace
 
@@ -608,7 +611,9 @@ class HostPool {
   private:
     HostPool() {
         const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-        for (int t = 0; t < std::min(7, hw - 1); ++t) th_.emplace_back([this] { loop(); });
+        // 14 workers: with the caller and a second host thread (the SCF's preparation beside its
+        // one-electron set-up) 16 threads, the CPU share of one GPU
+        for (int t = 0; t < std::min(14, hw - 1); ++t) th_.emplace_back([this] { loop(); });
     }
     void loop();
     std::vector<std::thread> th_;
@@ -649,7 +654,7 @@ void parallel_for(int n, int grain, F f) {
         return;
     }
     HostPool &pool = HostPool::get();
-    const int T = std::min({pool.workers() + 1, 8, n / std::max(grain, 1)});
+    const int T = std::min({pool.workers() + 1, 16, n / std::max(grain, 1)});
     std::unique_lock<std::mutex> use(pool_user, std::try_to_lock);
     if (T <= 1 || !use.owns_lock()) {
         f(0, n);
@@ -3447,14 +3452,21 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             if (t.joinable()) t.join();
         }
     } prep;
+    // the preparation enqueues on the first side stream (after the work already on the context's
+    // stream), so that neither thread's synchronisations wait for the other's kernels
+    STCHK(ensure_class_streams(ctx));
+    HIPCHK(hipEventRecord(ctx->fork_ev, S(ctx)));
+    HIPCHK(hipStreamWaitEvent(ctx->side[0], ctx->fork_ev, 0));
     prep.t = std::thread([&] {
+        tl_stream = ctx->side[0];
         prep.status = fock_jk_prepare(ctx, basis, opts.integral_policy, opts.screen, opts.schwarz_threshold, mixed ? 1 : 0);
+        tl_stream = nullptr;
     });
     Prepared &pr = ctx->jk;
 
     // device matrices
     DevBuf dS, dT, dV, dH, dX, dU, dTmp, dTmp2, dP, dG, dF, dFx, dC, dW, dE, dWork, dInfo, dErrs, dFocks, dDots,
-        dScal, dCoef, dSlots;
+        dScal, dCoef;
     for (DevBuf *b : {&dS, &dT, &dV, &dH, &dX, &dU, &dTmp, &dTmp2, &dP, &dG, &dF, &dFx, &dC}) HIPCHK(b->ensure(bytes));
     HIPCHK(dW.ensure(n * sizeof(double)));
     HIPCHK(dE.ensure(n * sizeof(double)));
@@ -3462,10 +3474,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     HIPCHK(dInfo.ensure(sizeof(int)));
     HIPCHK(dErrs.ensure(16 * bytes));
     HIPCHK(dFocks.ensure(16 * bytes));
-    HIPCHK(dDots.ensure(16 * sizeof(double)));
+    HIPCHK(dDots.ensure(17 * sizeof(double)));  // <e_new, e_slot> of the 16 ring slots, max|e_new|
     HIPCHK(dScal.ensure(4 * sizeof(double)));
-    HIPCHK(dCoef.ensure(16 * sizeof(double)));
-    HIPCHK(dSlots.ensure(16 * sizeof(int)));
+    HIPCHK(dCoef.ensure(16 * (sizeof(double) + sizeof(int))));  // DIIS coefficients, then their ring slots
     HIPCHK(fill_f64(16 * nn, 0.0, dErrs.as<double>(), S(ctx)));
     // EDIIS: the density of every stored Fock matrix, and per iteration tr(P_new F_j), tr(P_j F_new)
     // over the 16 ring slots and tr(P_new H); damping: the density before the step
@@ -3551,32 +3562,43 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     // max|lambda|: the bounds would only measure rounding noise, e.g. the GWH guess of H2, a
     // multiple of the identity). Those go to the eigensolver, whose choice within a degenerate
     // space is the reference's. Steps run in chunks without host round trips (one launch per
-    // step, the grow/shrink choice made on the device); the host looks at tr X - tr X^2 after each
-    // chunk.
-    constexpr int TC2_CHUNK = 8, TC2_MAX_STEPS = 96;
-    DevBuf dXp, dX2, dPur, dPart, dHist;
+    // step, the grow/shrink choice made on the device); after each chunk one copy brings the
+    // bounds and every step's (tr X, tr X^2) to pinned host memory, and the host looks at
+    // tr X - tr X^2. A round trip costs about seven steps, so the first chunk runs two steps past
+    // the count the previous purification needed (consecutive SCF iterations need similar counts;
+    // steps past convergence leave the projector as it is), the next ones 4, then 8; without a
+    // previous count (the guess) 32, then 16.
+    constexpr int TC2_MAX_STEPS = 96, TC2_REC = 2 + 3 * TC2_MAX_STEPS;
+    DevBuf dXp, dX2, dPart, dHist;
+    struct PinnedHist {
+        double *h = nullptr;
+        ~PinnedHist() {
+            if (h) (void)hipHostFree(h);
+        }
+    } pinh;
     const bool purify_ok = opts.integral_policy == FOCK_POLICY_ACCURATE;
     const int npart_max = std::max(tc2_init_parts(n), tc2_step_parts(n));
     if (purify_ok) {
         HIPCHK(dXp.ensure(bytes));
-        HIPCHK(dX2.ensure(bytes));
-        HIPCHK(dPur.ensure(8 * sizeof(double)));
+        HIPCHK(dX2.ensure(std::max(bytes, 2 * (size_t)n * sizeof(double))));  // also the row intervals
         HIPCHK(dPart.ensure(4 * (size_t)npart_max * sizeof(double)));
-        HIPCHK(dHist.ensure(3 * TC2_MAX_STEPS * sizeof(double)));
+        HIPCHK(dHist.ensure(TC2_REC * sizeof(double)));
+        HIPCHK(hipHostMalloc((void **)&pinh.h, TC2_REC * sizeof(double), hipHostMallocDefault));
     }
-    int tc2_last = 0;  // steps the previous purification took: the next one's first chunk
+    int tc2_last = 0;  // steps the previous purification needed (0: none yet)
     auto purify = [&](bool &ok) -> fock_status {
         ok = false;
-        double *pur = dPur.as<double>(), *hist = dHist.as<double>();  // pur[4..5]: lo, hi
+        // rec[0..1]: Gershgorin lo, hi; rec[2 + 3k ..]: tr X, tr X^2, grow of step k's input
+        double *rec = dHist.as<double>(), *hist = rec + 2;
         double *xin = dXp.as<double>(), *xout = dX2.as<double>();
         double *pin = dPart.as<double>(), *pout = pin + 2 * (size_t)npart_max;
-        HIPCHK(gershgorin_bounds(n, T2, pur + 4, S(ctx)));
-        HIPCHK(tc2_init(n, T2, pur + 4, xin, pin, S(ctx)));
+        HIPCHK(gershgorin_rows(n, T2, xout, S(ctx)));  // xout: free until the first step
+        HIPCHK(tc2_init(n, T2, xout, rec, xin, pin, S(ctx)));
         int npart = tc2_init_parts(n);
-        // consecutive SCF iterations need similar step counts: the first chunk runs one chunk
-        // short of the previous count (steps past convergence leave the projector as it is)
-        int chunk = std::max(TC2_CHUNK, tc2_last - TC2_CHUNK);
-        for (int it = 0; it < TC2_MAX_STEPS && !ok; it += chunk, chunk = TC2_CHUNK) {
+        const bool known = tc2_last > 0;
+        int chunk = known ? tc2_last + 2 : 32;
+        for (int it = 0, round = 0; it < TC2_MAX_STEPS && !ok; it += chunk, ++round) {
+            if (round > 0) chunk = known ? (round == 1 ? 4 : 8) : 16;
             chunk = std::min(chunk, TC2_MAX_STEPS - it);
             for (int k = 0; k < chunk; ++k) {
                 HIPCHK(tc2_step(n, xin, xout, pin, npart, pout, (double)nocc, hist + 3 * (it + k), S(ctx)));
@@ -3584,17 +3606,25 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 std::swap(pin, pout);
                 npart = tc2_step_parts(n);
             }
-            tc2_last = it + chunk;
-            double h[3], b[2];
-            HIPCHK(hipMemcpyAsync(h, hist + 3 * (it + chunk - 1), 3 * sizeof(double), hipMemcpyDeviceToHost,
+            const int done = it + chunk;
+            HIPCHK(hipMemcpyAsync(pinh.h, rec, (2 + 3 * (size_t)done) * sizeof(double), hipMemcpyDeviceToHost,
                                   S(ctx)));
-            HIPCHK(hipMemcpyAsync(b, pur + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
+            const double *b = pinh.h, *h = pinh.h + 2;
             if (!(b[1] - b[0] > 1e-8 * std::max(std::fabs(b[0]), std::fabs(b[1])))) return FOCK_OK;
-            const double idem = h[0] - h[1];  // of the input of the last step
+            const double *last = h + 3 * (done - 1);
+            const double idem = last[0] - last[1];  // of the input of the last step
             if (!std::isfinite(idem)) return FOCK_OK;
             // quadratic convergence: below 1e-9 before the last step means below 1e-18 after it
-            if (idem < 1e-9) ok = std::fabs(h[0] - nocc) < 1e-8;
+            if (idem < 1e-9) ok = std::fabs(last[0] - nocc) < 1e-8;
+            if (ok) {  // the count this purification needed: the first step whose input was converged, + 1
+                tc2_last = done;
+                for (int k = 0; k < done; ++k)
+                    if (h[3 * k] - h[3 * k + 1] < 1e-9 && std::fabs(h[3 * k] - nocc) < 1e-8) {
+                        tc2_last = k + 1;
+                        break;
+                    }
+            }
         }
         if (!ok) return FOCK_OK;
         HIPCHK(gemm_f64(false, false, n, n, n, 1.0, X_, n, xin, n, 0.0, T1, n, S(ctx)));  // X Xp
@@ -3636,6 +3666,8 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     phase_mark("guess");
     prep.t.join();
     STCHK(prep.status);
+    HIPCHK(hipEventRecord(ctx->join_ev[0], ctx->side[0]));
+    HIPCHK(hipStreamWaitEvent(S(ctx), ctx->join_ev[0], 0));
     phase_mark("jk_prepare");
     if (opts.autotune) {  // tiles for this molecule at the guess density, kept in the context
         fock_tune_report rep{};
@@ -3701,10 +3733,11 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             if (h) (void)hipHostFree(h);
         }
     } pinc;
-    HIPCHK(hipHostMalloc(&pinc.h, 18 * sizeof(double) + 16 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&pinc.h, 35 * sizeof(double) + 16 * sizeof(int), hipHostMallocDefault));
     double *pin_e = static_cast<double *>(pinc.h);  // [2]
-    double *pin_coef = pin_e + 2;                    // [16]
+    double *pin_coef = pin_e + 2;                    // [16], followed by the slots: one upload
     int *pin_slots = reinterpret_cast<int *>(pin_coef + 16);
+    double *pin_dots = reinterpret_cast<double *>(pin_slots + 16);  // [17]: the download of dDots
     std::vector<double> e_host;  // electronic energy of every iteration, read one iteration late
     int last_e = -1;             // last iteration whose energy was computed
     out->history_len = 0;
@@ -3765,8 +3798,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         double *f_slot = dFocks.as<double>() + (int64_t)slot * nn;
         HIPCHK(gemm_f64(false, false, n, n, n, 1.0, F_, n, P_, n, 0.0, T1, n, S(ctx)));   // F P
         HIPCHK(gemm_f64(false, false, n, n, n, 1.0, T1, n, S_, n, 0.0, T2, n, S(ctx)));   // F P S
-        HIPCHK(fill_f64(1, 0.0, scal, S(ctx)));
-        HIPCHK(antisym_maxabs(n, T2, e_slot, scal, S(ctx)));
+        double *errmax = dDots.as<double>() + 16;
+        HIPCHK(fill_f64(1, 0.0, errmax, S(ctx)));
+        HIPCHK(antisym_maxabs(n, T2, e_slot, errmax, S(ctx)));
         HIPCHK(copy_f64(nn, F_, f_slot, S(ctx)));
         if (ediis_on) {
             double *p_slot = dPs.as<double>() + (int64_t)slot * nn;
@@ -3793,9 +3827,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         {
             // dots against all 16 ring slots, pick the stored ones
             HIPCHK(frob_dots(nn, e_slot, dErrs.as<double>(), nn, 16, dDots.as<double>(), S(ctx)));
-            double hd[16 + 1], he[33];
-            HIPCHK(hipMemcpyAsync(hd, dDots.p, 16 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
-            HIPCHK(hipMemcpyAsync(hd + 16, scal, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+            double he[33];
+            const double *hd = pin_dots;
+            HIPCHK(hipMemcpyAsync(pin_dots, dDots.p, 17 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             if (ediis_on) HIPCHK(hipMemcpyAsync(he, dEd.p, 33 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
             if (last_e == iter - 1 && (int)e_host.size() == last_e) e_host.push_back(pin_e[last_e & 1]);
@@ -3829,9 +3863,10 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
                 pin_coef[i] = c[i];
                 pin_slots[i] = slots[i];
             }
-            HIPCHK(hipMemcpyAsync(dCoef.p, pin_coef, m * sizeof(double), hipMemcpyHostToDevice, S(ctx)));
-            HIPCHK(hipMemcpyAsync(dSlots.p, pin_slots, m * sizeof(int), hipMemcpyHostToDevice, S(ctx)));
-            HIPCHK(lincomb(nn, dFocks.as<double>(), nn, dSlots.as<int>(), dCoef.as<double>(), m, Fx_, S(ctx)));
+            HIPCHK(hipMemcpyAsync(dCoef.p, pin_coef, 16 * (sizeof(double) + sizeof(int)), hipMemcpyHostToDevice,
+                                  S(ctx)));
+            HIPCHK(lincomb(nn, dFocks.as<double>(), nn, reinterpret_cast<const int *>(dCoef.as<double>() + 16),
+                           dCoef.as<double>(), m, Fx_, S(ctx)));
             return FOCK_OK;
         };
         const bool exit_plain = opts.integral_policy == FOCK_POLICY_ACCURATE && phase == 1 && !incr_build &&

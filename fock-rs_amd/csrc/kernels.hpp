@@ -436,9 +436,13 @@ hipError_t gershgorin_bounds(int n, const double *A, double *out, hipStream_t s)
 hipError_t ns_init(int n, const double *S, const double *bound, double *Y, double *Z, hipStream_t s);
 hipError_t ns_update(int n, const double *ZY, double *T, double *res, hipStream_t s);
 hipError_t ns_finish(int n, const double *Z, const double *bound, double *X, hipStream_t s);
-// X = (hi I - A) / (hi - lo), per-workgroup (tr, sum of squares) partials in part[tc2_init_parts(n)][2]
+// Gershgorin interval of every row: rowb[2i], rowb[2i + 1]
+hipError_t gershgorin_rows(int n, const double *A, double *rowb, hipStream_t s);
+// {lo, hi} = the union of the row intervals (also written to bounds[0..1]), X = (hi I - A) / (hi - lo),
+// per-workgroup (tr, sum of squares) partials in part[tc2_init_parts(n)][2]
 int tc2_init_parts(int n);
-hipError_t tc2_init(int n, const double *A, const double *bounds, double *X, double *part, hipStream_t s);
+hipError_t tc2_init(int n, const double *A, const double *rowb, double *bounds, double *X, double *part,
+                    hipStream_t s);
 // one purification step X -> Xout (X^2 or 2X - X^2, chosen on the device from part_in), the new
 // partials in part_out[tc2_step_parts(n)][2], hist = {tr X, tr X^2, grow} of the input
 int tc2_step_parts(int n);
