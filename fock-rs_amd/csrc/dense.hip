@@ -263,15 +263,22 @@ __device__ __forceinline__ void energy_trace_kernel_body(int n, const double *__
     const double v = block_reduce<NT>((int64_t)n * n, [&](int64_t e) { return P[e] * (2.0 * H[e] + G[e]); });
     if (threadIdx.x == 0) out[0] = 0.5 * v;
 }
-__global__ __launch_bounds__(REDUCE_NT) void energy_trace_kernel(int n, const double *__restrict__ P,
-                                                                 const double *__restrict__ H,
-                                                                 const double *__restrict__ G,
-                                                                 double *__restrict__ out) {
-    energy_trace_kernel_body<REDUCE_NT>(n, P, H, G, out);
-}
 
-hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(energy_trace_kernel, dim3(1), dim3(REDUCE_NT), 0, s, n, P, H, G, out);
+// E_elec split over ENERGY_PARTS workgroups (each a contiguous range of the n^2 products):
+// parts[b] = 1/2 sum over range b of P (2H + G); the caller adds the parts in order
+__global__ __launch_bounds__(256) void energy_parts_kernel(int n, const double *__restrict__ P,
+                                                           const double *__restrict__ H,
+                                                           const double *__restrict__ G, double *__restrict__ parts) {
+    const int64_t len = (int64_t)n * n;
+    const int64_t e0 = len * blockIdx.x / ENERGY_PARTS, e1 = len * (blockIdx.x + 1) / ENERGY_PARTS;
+    const double v = block_reduce<256>(e1 - e0, [&](int64_t i) {
+        const int64_t e = e0 + i;
+        return P[e] * (2.0 * H[e] + G[e]);
+    });
+    if (threadIdx.x == 0) parts[blockIdx.x] = 0.5 * v;
+}
+hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *parts, hipStream_t s) {
+    hipLaunchKernelGGL(energy_parts_kernel, dim3(ENERGY_PARTS), dim3(256), 0, s, n, P, H, G, parts);
     return hipGetLastError();
 }
 

@@ -663,6 +663,35 @@ void parallel_for(int n, int grain, F f) {
     pool.run(T, [&](int t) { f((int)((int64_t)n * t / T), (int)((int64_t)n * (t + 1) / T)); });
 }
 
+// Stable order of [0, n) by the 128-bit key (hi[i], lo[i]) ascending: least-significant-digit
+// radix sort over 11-bit digits, digits that every key shares skipped. Ties keep index order.
+void radix_order(const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi, std::vector<int> &order) {
+    const int n = (int)lo.size();
+    std::vector<int> tmp(n);
+    order.resize(n);
+    std::iota(order.begin(), order.end(), 0);
+    uint32_t cnt[2048];
+    for (int w = 0; w < 2; ++w) {
+        const std::vector<uint64_t> &key = w == 0 ? lo : hi;
+        for (int sh = 0; sh < 64; sh += 11) {
+            std::memset(cnt, 0, sizeof(cnt));
+            for (int i = 0; i < n; ++i) ++cnt[(key[i] >> sh) & 2047];
+            if (n == 0 || cnt[(key[0] >> sh) & 2047] == (uint32_t)n) continue;  // one digit value: order unchanged
+            uint32_t run = 0;
+            for (uint32_t &c : cnt) {
+                const uint32_t t = c;
+                c = run;
+                run += t;
+            }
+            for (int i = 0; i < n; ++i) {
+                const int e = order[i];
+                tmp[cnt[(key[e] >> sh) & 2047]++] = e;
+            }
+            order.swap(tmp);
+        }
+    }
+}
+
 // Blocks of class cls at a tile: runs of up to `tile` consecutive tasks of one bra pair (task
 // indices class-local), most expensive first (load balance; stable for determinism).
 void make_class_blocks(const Prepared &pr, int cls, int tile, std::vector<Block> &out) {
@@ -887,38 +916,29 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
     }
 
     mark("diag_D");
-    // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation). The
-    // keys are sorted by value (no indirection into the pair records), the two sorts side by side
-    struct SortKey {
-        int cls, K;
-        double dmax;
-        int creation, idx;
-    };
-    std::vector<SortKey> ko(npair), kr;
-    for (int i = 0; i < npair; ++i) ko[i] = SortKey{pairs[i].cls, pairs[i].K, pairs[i].dmax, pairs[i].creation, i};
-    kr = ko;
+    // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation). Pairs
+    // are created in index order and the radix sorts are stable, so creation breaks the ties by
+    // itself; dmax >= 0, whose bit pattern orders like its value (complemented: descending). The
+    // two sorts side by side.
+    std::vector<uint64_t> kd(npair), kck(npair), kc(npair);
+    for (int i = 0; i < npair; ++i) {
+        uint64_t bits;
+        std::memcpy(&bits, &pairs[i].dmax, sizeof(bits));
+        kd[i] = ~bits;
+        kc[i] = (uint64_t)pairs[i].cls;
+        kck[i] = ((uint64_t)pairs[i].cls << 32) | (uint32_t)pairs[i].K;
+    }
+    std::vector<int> order, by_rank;
     parallel_for(2, 1, [&](int lo, int hi) {
         for (int which = lo; which < hi; ++which) {
             if (which == 0)
-                std::sort(ko.begin(), ko.end(), [](const SortKey &a, const SortKey &b) {
-                    if (a.cls != b.cls) return a.cls < b.cls;
-                    if (a.K != b.K) return a.K < b.K;
-                    if (a.dmax != b.dmax) return a.dmax > b.dmax;
-                    return a.creation < b.creation;
-                });
+                radix_order(kd, kck, order);
             else
-                std::sort(kr.begin(), kr.end(), [](const SortKey &a, const SortKey &b) {
-                    if (a.cls != b.cls) return a.cls < b.cls;
-                    if (a.dmax != b.dmax) return a.dmax > b.dmax;
-                    return a.creation < b.creation;
-                });
+                radix_order(kd, kc, by_rank);
         }
     });
-    std::vector<int> order(npair), rank_in_class(npair);
-    for (int i = 0; i < npair; ++i) {
-        order[i] = ko[i].idx;
-        rank_in_class[kr[i].idx] = i;
-    }
+    std::vector<int> rank_in_class(npair);
+    for (int i = 0; i < npair; ++i) rank_in_class[by_rank[i]] = i;
     pr.pairs.resize(npair);
     std::vector<PairInfo> info(npair);
     std::vector<int> rank_sorted(npair);
@@ -3475,7 +3495,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     HIPCHK(dErrs.ensure(16 * bytes));
     HIPCHK(dFocks.ensure(16 * bytes));
     HIPCHK(dDots.ensure(17 * sizeof(double)));  // <e_new, e_slot> of the 16 ring slots, max|e_new|
-    HIPCHK(dScal.ensure(4 * sizeof(double)));
+    HIPCHK(dScal.ensure((2 + ENERGY_PARTS) * sizeof(double)));
     HIPCHK(dCoef.ensure(16 * (sizeof(double) + sizeof(int))));  // DIIS coefficients, then their ring slots
     HIPCHK(fill_f64(16 * nn, 0.0, dErrs.as<double>(), S(ctx)));
     // EDIIS: the density of every stored Fock matrix, and per iteration tr(P_new F_j), tr(P_j F_new)
@@ -3733,9 +3753,14 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             if (h) (void)hipHostFree(h);
         }
     } pinc;
-    HIPCHK(hipHostMalloc(&pinc.h, 35 * sizeof(double) + 16 * sizeof(int), hipHostMallocDefault));
-    double *pin_e = static_cast<double *>(pinc.h);  // [2]
-    double *pin_coef = pin_e + 2;                    // [16], followed by the slots: one upload
+    HIPCHK(hipHostMalloc(&pinc.h, (2 * ENERGY_PARTS + 33) * sizeof(double) + 16 * sizeof(int), hipHostMallocDefault));
+    double *pin_e = static_cast<double *>(pinc.h);  // [2][ENERGY_PARTS]
+    auto energy_of = [&](int slot) {                 // the parts added in order
+        double e = 0.0;
+        for (int k = 0; k < ENERGY_PARTS; ++k) e += pin_e[slot * ENERGY_PARTS + k];
+        return e;
+    };
+    double *pin_coef = pin_e + 2 * ENERGY_PARTS;     // [16], followed by the slots: one upload
     int *pin_slots = reinterpret_cast<int *>(pin_coef + 16);
     double *pin_dots = reinterpret_cast<double *>(pin_slots + 16);  // [17]: the download of dDots
     std::vector<double> e_host;  // electronic energy of every iteration, read one iteration late
@@ -3832,7 +3857,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             HIPCHK(hipMemcpyAsync(pin_dots, dDots.p, 17 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             if (ediis_on) HIPCHK(hipMemcpyAsync(he, dEd.p, 33 * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
             HIPCHK(hipStreamSynchronize(S(ctx)));
-            if (last_e == iter - 1 && (int)e_host.size() == last_e) e_host.push_back(pin_e[last_e & 1]);
+            if (last_e == iter - 1 && (int)e_host.size() == last_e) e_host.push_back(energy_of(last_e & 1));
             for (int i = 0; i < m; ++i) newdots[i] = hd[slots[i]];
             out->last_error = hd[16];
             if (ediis_on) {  // the new row (P_new F_j) and column (P_j F_new) of tr(P_i F_j)
@@ -3914,8 +3939,9 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         // P = F P_new + (1 - F) P_old (scf/src/lib.rs:218-219, F = opts.damping)
         if (damp) HIPCHK(mix_f64(nn, opts.damping, P_, dPold.as<double>(), P_, S(ctx)));
         // E_elec = 1/2 tr(P_new (2H + G(P_old))) (:223)
-        HIPCHK(energy_trace(n, P_, H_, G_, scal + 1, S(ctx)));
-        HIPCHK(hipMemcpyAsync(pin_e + (iter & 1), scal + 1, sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
+        HIPCHK(energy_trace(n, P_, H_, G_, scal + 2, S(ctx)));
+        HIPCHK(hipMemcpyAsync(pin_e + (iter & 1) * ENERGY_PARTS, scal + 2, ENERGY_PARTS * sizeof(double),
+                              hipMemcpyDeviceToHost, S(ctx)));
         last_e = iter;
         phase_mark("post_eig");
         float ms = 0.0f;
@@ -3929,7 +3955,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
         if (debug) {
             HIPCHK(hipStreamSynchronize(S(ctx)));
             std::fprintf(stderr, "[fock] iter %d phase %d err %.6e E %.12f build %.3f ms quartets %lld\n", iter, phase,
-                         err, pin_e[iter & 1] + e_nuc, ms, (long long)evaluated);
+                         err, energy_of(iter & 1) + e_nuc, ms, (long long)evaluated);
         }
         out->iterations = iter;
         if (phase == 0) {
@@ -3984,7 +4010,7 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     out->n_basis = n;
     HIPCHK(hipStreamSynchronize(S(ctx)));
     if (last_e >= 0) {  // the energies of the iterations (history) and of the last one
-        if ((int)e_host.size() == last_e) e_host.push_back(pin_e[last_e & 1]);
+        if ((int)e_host.size() == last_e) e_host.push_back(energy_of(last_e & 1));
         for (int k = 0; k < out->history_len && k <= last_e; ++k) out->history[k].total_energy = e_host[k] + e_nuc;
         out->electronic_energy = e_host[last_e];
         out->total_energy = e_host[last_e] + e_nuc;

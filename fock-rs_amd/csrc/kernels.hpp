@@ -417,8 +417,9 @@ hipError_t lincomb(int64_t len, const double *base, int64_t stride, const int *s
                    double *out, hipStream_t s);
 // out[i][j] = 2 * sum_{k<nocc} C[i][k] C[j][k]  (C row-major n x n)
 hipError_t density_from_coeffs(int n, int nocc, const double *C, double *P, hipStream_t s);
-// *out = 0.5 * sum_ij P_ij (2 H_ij + G_ij)
-hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *out, hipStream_t s);
+// 0.5 * sum_ij P_ij (2 H_ij + G_ij) = the sum of parts[0 .. ENERGY_PARTS), added in order
+constexpr int ENERGY_PARTS = 32;
+hipError_t energy_trace(int n, const double *P, const double *H, const double *G, double *parts, hipStream_t s);
 // x = a - b elementwise (len)
 hipError_t sub_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s);
 hipError_t add_f64(int64_t len, const double *a, const double *b, double *out, hipStream_t s);
