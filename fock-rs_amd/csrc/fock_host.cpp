@@ -3720,6 +3720,20 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     for (int c = 0; c < NUM_CLASSES; ++c) static_quartets += pr.stats.shell_quartets[c];
     DevBuf bGacc, bPprev, bPinc;
     double *Gacc = nullptr, *Pprev = nullptr, *dPinc = nullptr;
+    // kept-ket counts of the blocked increments (history only): copied into pinned memory after
+    // each build and read after the loop, without a synchronisation of their own
+    struct PinnedKept {
+        unsigned long long *h = nullptr;
+        ~PinnedKept() {
+            if (h) (void)hipHostFree(h);
+        }
+    } pink;
+    std::vector<char> kept_async;  // iterations whose count is in pink
+    if (incremental && want_counts && !debug) {
+        HIPCHK(hipHostMalloc((void **)&pink.h, (size_t)(opts.max_iters + 1) * NUM_CLASSES * sizeof(unsigned long long),
+                             hipHostMallocDefault));
+        kept_async.assign(opts.max_iters + 1, 0);
+    }
     if (incremental) {
         HIPCHK(bGacc.ensure(bytes));
         HIPCHK(bPprev.ensure(bytes));
@@ -3782,7 +3796,11 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
             if (blocked_fits_screened(ctx, n)) {
                 STCHK(enqueue_jk_incremental_blocked(ctx, dPinc, pr.d_Jb.as<double>(), pr.d_Kb.as<double>(), tau,
                                                      tau64, want_counts));
-                if (want_counts) {
+                if (want_counts && pink.h) {
+                    HIPCHK(hipMemcpyAsync(pink.h + (size_t)iter * NUM_CLASSES, pr.d_kept.p,
+                                          NUM_CLASSES * sizeof(unsigned long long), hipMemcpyDeviceToHost, S(ctx)));
+                    kept_async[iter] = 1;
+                } else if (want_counts) {
                     unsigned long long kc[NUM_CLASSES] = {};
                     HIPCHK(hipMemcpyAsync(kc, pr.d_kept.p, sizeof(kc), hipMemcpyDeviceToHost, S(ctx)));
                     HIPCHK(hipStreamSynchronize(S(ctx)));
@@ -4012,6 +4030,14 @@ extern "C" fock_status fock_scf_run(fock_ctx *ctx, const fock_basis_desc *basis,
     if (last_e >= 0) {  // the energies of the iterations (history) and of the last one
         if ((int)e_host.size() == last_e) e_host.push_back(energy_of(last_e & 1));
         for (int k = 0; k < out->history_len && k <= last_e; ++k) out->history[k].total_energy = e_host[k] + e_nuc;
+    }
+    for (int k = 0; k < out->history_len && k < (int)kept_async.size(); ++k)
+        if (kept_async[k]) {
+            int64_t kept = 0;
+            for (int c = 0; c < NUM_CLASSES; ++c) kept += (int64_t)pink.h[(size_t)k * NUM_CLASSES + c];
+            out->history[k].quartets = kept;
+        }
+    if (last_e >= 0) {
         out->electronic_energy = e_host[last_e];
         out->total_energy = e_host[last_e] + e_nuc;
     }
