@@ -24,6 +24,7 @@
 #include <atomic>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -100,6 +101,40 @@ struct DevBuf {
     }
 };
 
+// Host arrays whose contents go to the device in one copy (primitive pairs, task and block
+// lists). A context's own preparation keeps them page-locked, so that the copy is a DMA transfer
+// instead of one staged through a bounce buffer while the calling thread waits; the batch's
+// per-molecule preparations keep ordinary memory (many small page-locked allocations, and the
+// device synchronisation of freeing them, cost more than their copies gain).
+template <class T>
+struct HostAlloc {
+    using value_type = T;
+    using propagate_on_container_move_assignment = std::true_type;
+    using propagate_on_container_swap = std::true_type;
+    bool pin = false;
+    HostAlloc() = default;
+    explicit HostAlloc(bool p) : pin(p) {}
+    template <class U>
+    HostAlloc(const HostAlloc<U> &o) : pin(o.pin) {}
+    T *allocate(size_t count) {
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (!pin) return static_cast<T *>(::operator new(bytes));
+        void *p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t) {
+        if (pin) (void)hipHostFree(p);
+        else ::operator delete(p);
+    }
+    template <class U>
+    bool operator==(const HostAlloc<U> &o) const { return pin == o.pin; }
+    template <class U>
+    bool operator!=(const HostAlloc<U> &o) const { return pin != o.pin; }
+};
+template <class T>
+using HostVec = std::vector<T, HostAlloc<T>>;
+
 struct Shell {
     int L, f0, nf, K, prim0;  // prim0: index of the first primitive of function f0
     double A[3];
@@ -126,13 +161,13 @@ struct Prepared {
     bool prim_counted = false;  // stats.primitive_quartets holds the kept (screened) count
     int task_off[NUM_CLASSES + 1] = {0};
     int block_off[NUM_CLASSES + 1] = {0};
-    std::vector<Task> h_tasks;    // the task lists of d_tasks (block lists are rebuilt from them)
+    HostVec<Task> h_tasks;      // the task lists of d_tasks (block lists are rebuilt from them)
     // host arrays of the preparation, kept so that a preparation of the same size writes into
     // memory that is already mapped (fresh pages cost a fault each)
-    std::vector<int> h_gfirst, h_glast;
-    std::vector<PrimPair<double>> h_prims;
-    std::vector<double2> h_prim_ab;
-    std::vector<Block> h_blocks;  // the block lists of d_blocks
+    HostVec<int> h_gfirst, h_glast;
+    HostVec<PrimPair<double>> h_prims;
+    HostVec<double2> h_prim_ab;
+    HostVec<Block> h_blocks;  // the block lists of d_blocks
     int tile[NUM_CLASSES] = {DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE};
     fock_jk_stats stats{};
     DevBuf d_Jb, d_Kb, d_P, d_J, d_K;
@@ -155,6 +190,15 @@ struct Prepared {
     std::vector<double> key_d;
     std::vector<int32_t> key_i;
     uint64_t pair_hash = 0;  // FNV-1a of the sorted pair records and the primitive pairs' q
+    // page-locked host arrays from now on (empty arrays only: called before the first preparation)
+    void pin_host_arrays() {
+        h_tasks = HostVec<Task>(HostAlloc<Task>(true));
+        h_gfirst = HostVec<int>(HostAlloc<int>(true));
+        h_glast = HostVec<int>(HostAlloc<int>(true));
+        h_prims = HostVec<PrimPair<double>>(HostAlloc<PrimPair<double>>(true));
+        h_prim_ab = HostVec<double2>(HostAlloc<double2>(true));
+        h_blocks = HostVec<Block>(HostAlloc<Block>(true));
+    }
 };
 
 // FNV-1a over bytes, continuing from h.
@@ -436,6 +480,7 @@ extern "C" fock_status fock_ctx_create(int device, fock_ctx **out) {
     for (auto &form : ctx->tiles)
         for (int &t : form) t = DEFAULT_TILE;
     HIPCHK(hipSetDevice(device));
+    ctx->jk.pin_host_arrays();
     {
         int lds = 0;
         HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -717,6 +762,9 @@ fock_status upload_blocks(fock_ctx *ctx, Prepared &pr) {
         for (int cls = lo; cls < hi; ++cls) make_class_blocks(pr, cls, pr.tile[cls], cb[cls]);
     });
     pr.h_blocks.clear();
+    size_t total = 0;
+    for (int cls = 0; cls < NUM_CLASSES; ++cls) total += cb[cls].size();
+    pr.h_blocks.reserve(total);  // one (pinned) allocation at most
     for (int cls = 0; cls < NUM_CLASSES; ++cls) {
         pr.block_off[cls] = (int)pr.h_blocks.size();
         pr.h_blocks.insert(pr.h_blocks.end(), cb[cls].begin(), cb[cls].end());
@@ -794,8 +842,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
             pairs.push_back(ph);
         }
     // primitive pairs of every shell pair (a-major), filled in parallel at their offsets
-    std::vector<PrimPair<double>> &prims = pr.h_prims;
-    std::vector<double2> &prim_ab = pr.h_prim_ab;  // exponents of the two primitives of every primitive pair
+    HostVec<PrimPair<double>> &prims = pr.h_prims;
+    HostVec<double2> &prim_ab = pr.h_prim_ab;  // exponents of the two primitives of every primitive pair
     prims.resize(nprim_all);
     prim_ab.resize(nprim_all);
     parallel_for((int)pairs.size(), 256, [&](int lo, int hi) {
@@ -1010,8 +1058,8 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
         return di * dj > pr.dthr;
     };
     mark("soa");
-    std::vector<Task> &tasks = pr.h_tasks;
-    std::vector<int> &gfirst_all = pr.h_gfirst, &glast_all = pr.h_glast;
+    HostVec<Task> &tasks = pr.h_tasks;
+    HostVec<int> &gfirst_all = pr.h_gfirst, &glast_all = pr.h_glast;
     fock_jk_stats st{};
     st.n_basis = n;
     st.n_shells = ns;
@@ -1639,8 +1687,9 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
                                ms, 5);
         };
         STCHK(time_build(best_build));
+        int seed_best[NUM_CLASSES];
         {
-            int per_class[NUM_CLASSES], seed_best[NUM_CLASSES];
+            int per_class[NUM_CLASSES];
             for (int c = 0; c < NUM_CLASSES; ++c) per_class[c] = seed_best[c] = pr.tile[c];
             for (int seed : {32, 64}) {
                 for (int c = 0; c < NUM_CLASSES; ++c) {
@@ -1688,6 +1737,24 @@ fock_status tune_prepared(fock_ctx *ctx, const double *dP, fock_tune_report *rep
                 if (keep != start) moved = true;
             }
             if (!moved) break;
+        }
+        // every comparison above is against the least of the earlier noisy timings, so a move can
+        // be a lucky draw: the final tiles and the best seed are timed again, alternately, three
+        // times each, and the faster kept
+        if (!std::equal(pr.tile, pr.tile + NUM_CLASSES, seed_best)) {
+            int fin[NUM_CLASSES];
+            std::copy(pr.tile, pr.tile + NUM_CLASSES, fin);
+            double t_fin = HUGE_VAL, t_seed = HUGE_VAL;
+            for (int r = 0; r < 3; ++r) {
+                double ms = 0.0;
+                std::copy(fin, fin + NUM_CLASSES, pr.tile);
+                STCHK(time_build(ms));
+                t_fin = std::min(t_fin, ms);
+                std::copy(seed_best, seed_best + NUM_CLASSES, pr.tile);
+                STCHK(time_build(ms));
+                t_seed = std::min(t_seed, ms);
+            }
+            std::copy(t_fin <= t_seed ? fin : seed_best, (t_fin <= t_seed ? fin : seed_best) + NUM_CLASSES, pr.tile);
         }
         STCHK(upload_blocks(ctx, pr));
     }

@@ -378,8 +378,9 @@ typedef struct fock_tune_report {
  * the per-class choice is then refined on whole builds: starting from the fastest of the per-class
  * choice, every class at 32 and every class at 64 (where they fit), each class, longest first,
  * tries every tile with the others fixed and keeps the one with the fastest whole build (timed,
- * best of five), for up to two passes (report->ms holds the per-class times, report->tile the final
- * choice). report may be NULL. */
+ * best of five), for up to two passes; the result and the best start are then timed again side by
+ * side and the faster kept (report->ms holds the per-class times, report->tile the final choice).
+ * report may be NULL. */
 fock_status fock_jk_autotune(fock_ctx *ctx, const double *P, fock_tune_report *report);
 /* Read or set the context's tiles of a form (fock_tile_form); tiles are 1, 2, 4, 8, 16, 32 or 64 (default
  * 8). schedule applies to FOCK_TILES_BATCH (fock_batch_schedule; default COST). tile may be NULL. */
