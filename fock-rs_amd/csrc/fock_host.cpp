@@ -168,6 +168,7 @@ struct Prepared {
     HostVec<PrimPair<double>> h_prims;
     HostVec<double2> h_prim_ab;
     HostVec<Block> h_blocks;  // the block lists of d_blocks
+    HostVec<double> h_D;      // diagonal ERIs (ab|ab) of every basis-function pair, n x n
     int tile[NUM_CLASSES] = {DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE, DEFAULT_TILE};
     fock_jk_stats stats{};
     DevBuf d_Jb, d_Kb, d_P, d_J, d_K;
@@ -198,6 +199,7 @@ struct Prepared {
         h_prims = HostVec<PrimPair<double>>(HostAlloc<PrimPair<double>>(true));
         h_prim_ab = HostVec<double2>(HostAlloc<double2>(true));
         h_blocks = HostVec<Block>(HostAlloc<Block>(true));
+        h_D = HostVec<double>(HostAlloc<double>(true));
     }
 };
 
@@ -751,7 +753,15 @@ void make_class_blocks(const Prepared &pr, int cls, int tile, std::vector<Block>
         out.push_back(Block{q - t0, r - q, pr.h_tasks[q].bra, cost});
         q = r;
     }
-    std::stable_sort(out.begin() + first, out.end(), [](const Block &a, const Block &b) { return a.cost > b.cost; });
+    // by cost, descending, stable (cost >= 0: its complement in 32 bits orders descending)
+    const size_t cnt = out.size() - first;
+    std::vector<uint64_t> lo(cnt), hi(cnt, 0);
+    for (size_t i = 0; i < cnt; ++i) lo[i] = 0xffffffffull - (uint32_t)out[first + i].cost;
+    std::vector<int> order;
+    radix_order(lo, hi, order);
+    std::vector<Block> sorted(cnt);
+    for (size_t i = 0; i < cnt; ++i) sorted[i] = out[first + order[i]];
+    std::copy(sorted.begin(), sorted.end(), out.begin() + first);
 }
 
 // Block lists of every class at the prepared tiles into d_blocks (room for tile 1 is kept, so
@@ -952,16 +962,20 @@ fock_status prepare_into(fock_ctx *ctx, Prepared &pr, const fock_basis_desc *bas
             start += cnt;
         }
     }
-    std::vector<double> D((size_t)n * n);
+    HostVec<double> &D = pr.h_D;
+    D.resize((size_t)n * n);
     HIPCHK(hipMemcpyAsync(D.data(), pr.d_D.p, D.size() * sizeof(double), hipMemcpyDeviceToHost, S(ctx)));
     HIPCHK(hipStreamSynchronize(S(ctx)));
-    for (PairHost &ph : pairs) {
-        const Shell &A = pr.shells[ph.sa], &B = pr.shells[ph.sb];
-        double m = 0.0;
-        for (int i = 0; i < A.nf; ++i)
-            for (int j = 0; j < B.nf; ++j) m = std::max(m, D[(size_t)(A.f0 + i) * n + (B.f0 + j)]);
-        ph.dmax = std::isnan(m) ? HUGE_VAL : m;  // NaN inputs: keep the pair, sort stays well defined
-    }
+    parallel_for((int)pairs.size(), 1024, [&](int lo, int hi) {
+        for (int t = lo; t < hi; ++t) {
+            PairHost &ph = pairs[t];
+            const Shell &A = pr.shells[ph.sa], &B = pr.shells[ph.sb];
+            double m = 0.0;
+            for (int i = 0; i < A.nf; ++i)
+                for (int j = 0; j < B.nf; ++j) m = std::max(m, D[(size_t)(A.f0 + i) * n + (B.f0 + j)]);
+            ph.dmax = std::isnan(m) ? HUGE_VAL : m;  // NaN inputs: keep the pair, sort stays well defined
+        }
+    });
 
     mark("diag_D");
     // task order: (class, K, dmax desc, creation); rank within class: (dmax desc, creation). Pairs
